@@ -1,0 +1,266 @@
+/*
+ * sparrow_hip: MI355X (gfx950) kernels for sparrowpy's directional
+ * radiosity hot path, behind a plain C ABI.
+ *
+ * Conventions
+ *  - Every pointer argument is a DEVICE pointer unless named *_host.
+ *  - Arrays are C-contiguous with the shapes given in brackets.
+ *  - Geometry is float64; visibility flags are uint8 (0/1); patch and
+ *    direction indices are int32 or int64 as stated; energies are float32.
+ *  - `stream` is a hipStream_t (NULL = default stream).  Calls are
+ *    asynchronous unless documented otherwise; buffers are caller-owned and
+ *    no entry point allocates device memory.
+ *  - Return value: SPR_OK (0) or a negative SPR_ERR_* code; the message is
+ *    available from spr_last_error() on the calling thread.
+ *
+ * Each entry point names the reference function it replaces
+ * (sparrowpy 1.0.1 file:line).
+ */
+#ifndef SPARROW_HIP_H
+#define SPARROW_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SPR_OK 0
+#define SPR_ERR_ARG (-1)         /* bad shape, size or pointer */
+#define SPR_ERR_HIP (-2)         /* HIP runtime error */
+#define SPR_ERR_RANGE (-3)       /* a delay falls outside the histogram */
+#define SPR_ERR_UNSUPPORTED (-4) /* e.g. more directions than supported */
+#define SPR_ERR_WORKSPACE (-5)   /* workspace too small */
+
+/* Message of the last failing call on this thread ("" if none). */
+const char *spr_last_error(void);
+
+/* Library version string. */
+const char *spr_version(void);
+
+/* ---------------------------------------------------------------------
+ * Bake: visibility
+ * ------------------------------------------------------------------- */
+
+/* geometry._check_patch2patch_visibility (sparrowpy/geometry.py:750-797),
+ * called by DirectionalRadiosityFast.bake_geometry (RadiosityFast.py:374).
+ * centers[P,3]; surfaces (possible blockers) surf_normals[NS,3],
+ * surf_points[NS,4,3].  vis[P,P]: vis[i,j] for i<j is 1 when no surface
+ * blocks the segment between the centres, 0 otherwise; the diagonal and the
+ * lower triangle are 0.  Bit-exact with the reference predicate. */
+int spr_p2p_visibility(int64_t P, const double *centers, int64_t NS,
+                       const double *surf_normals, const double *surf_points,
+                       uint8_t *vis, void *workspace, int64_t workspace_bytes,
+                       void *stream);
+
+/* Workspace bytes of spr_p2p_visibility / spr_point_visibility for NS
+ * blocking surfaces (one precomputed record per surface). */
+int64_t spr_visibility_workspace(int64_t NS);
+
+/* geometry._check_point2patch_visibility (sparrowpy/geometry.py:799-839),
+ * called by init_source_energy (RadiosityFast.py:482) and
+ * _collect_energy_patches (RadiosityFast.py:714), batched over NQ query
+ * points.  points[NQ,3], centers[P,3], surf_normals[NS,3],
+ * surf_points[NS,4,3] -> vis[NQ,P]. */
+int spr_point_visibility(int64_t NQ, const double *points, int64_t P,
+                         const double *centers, int64_t NS,
+                         const double *surf_normals,
+                         const double *surf_points, uint8_t *vis,
+                         void *workspace, int64_t workspace_bytes,
+                         void *stream);
+
+/* Visible-pair list, the loop at RadiosityFast.py:377-387.
+ * Pass 1 (pairs == NULL): writes row_offsets[P+1] (exclusive scan of the
+ * per-row counts of vis[i, j>i]); row_offsets[P] is the pair count V.
+ * Pass 2 (pairs != NULL, row_offsets from pass 1): writes pairs[V,2]
+ * (int32, row-major i<j, the reference order). */
+int spr_visible_pairs(int64_t P, const uint8_t *vis, int64_t *row_offsets,
+                      int32_t *pairs, void *stream);
+
+/* ---------------------------------------------------------------------
+ * Bake: form factors and direction maps
+ * ------------------------------------------------------------------- */
+
+/* form_factor.patch2patch_ff_universal (sparrowpy/form_factor/
+ * universal.py:12-52): for each pair (i,j) in pairs[V,2], ff[i,j] =
+ * universal_form_factor(patch i -> patch j) (Nusselt analogue with 64
+ * regular samples when the patches share a vertex, Stokes contour
+ * integration otherwise).  points[P,4,3], normals[P,3], areas[P].
+ * ff[P,P] is zero-filled first. */
+int spr_form_factors(int64_t P, const double *points, const double *normals,
+                     const double *areas, int64_t V, const int32_t *pairs,
+                     double *ff, void *stream);
+
+/* get_scattering_data_receiver_index (RadiosityFast.py:1277-1312) as used
+ * to build the patch-to-outgoing-direction map (RadiosityFast.py:403-414):
+ * p2o[i,j] = argmin_d |out_dirs[wall_i, d] - unit(c_j - c_i)|^2 when
+ * (i,j) is visible in either orientation, D otherwise.  vis[P,P] (upper
+ * triangle), centers[P,3], out_dirs[W,D,3], wall_ids[P] (int64) ->
+ * p2o[P,P] (int64). */
+int spr_direction_map(int64_t P, const uint8_t *vis, const double *centers,
+                      int64_t D, const double *out_dirs,
+                      const int64_t *wall_ids, int64_t *p2o, void *stream);
+
+/* get_scattering_data_receiver_index for one target point (receiver
+ * collection, RadiosityFast.py:728-730): idx[k] = argmin_d
+ * |out_dirs[wall_k, d] - unit(point - c_k)|^2 for every patch k.
+ * points[NQ,3] -> idx[NQ,P] (int64). */
+int spr_point_direction_index(int64_t NQ, const double *points, int64_t P,
+                              const double *centers, int64_t D,
+                              const double *out_dirs,
+                              const int64_t *wall_ids, int64_t *idx,
+                              void *stream);
+
+/* _form_factors_with_directivity_dim (RadiosityFast.py:1234-1272), dense:
+ * fft[i,j,d,b] = F~_ij * exp(-air_b * |u_ij|) * brdf[bi(w_i), s_ij, d, b]
+ * for visible (i,j), 0 elsewhere, with F~_ij = ff[i,j] (i<j) or
+ * ff[j,i]*A_j/A_i, u_ij = unit(c_i - c_j) and s_ij the nearest incoming
+ * direction of wall w_i to u_ij.  air may be NULL (no air factor); brdf may
+ * be NULL (then D = 1 and no BRDF factor).  vis[P,P], ff[P,P],
+ * centers[P,3], areas[P], air[B], wall_ids[P], brdf[NBRDF,S,D,B],
+ * brdf_index[W] (int64), in_dirs[W,S,3] -> fft[P,P,D,B] (float64). */
+int spr_fft_dense(int64_t P, const uint8_t *vis, const double *ff, int64_t B,
+                  const double *centers, const double *areas,
+                  const double *air, const int64_t *wall_ids, int64_t S,
+                  int64_t D, const double *brdf, const int64_t *brdf_index,
+                  const double *in_dirs, double *fft, void *stream);
+
+/* ---------------------------------------------------------------------
+ * Source injection
+ * ------------------------------------------------------------------- */
+
+/* _source2patch_energy_universal (universal.py:98-147) followed by
+ * _add_directional (RadiosityFast.py:988-1034), the body of
+ * init_source_energy (RadiosityFast.py:482-497).  source[3], centers[P,3],
+ * points[P,4,3], vis[P] (source visibility), air[B] (may be NULL),
+ * wall_ids[P], brdf[NBRDF,S,D,B], brdf_index[W], in_dirs[W,S,3] ->
+ * energy0[P,B] (float64, before the BRDF), e0dir[P,D,B] (float64),
+ * distance[P] (float64; 0 for patches the source does not see). */
+int spr_source_energy(const double *source, int64_t P, const double *centers,
+                      const double *points, const uint8_t *vis,
+                      const double *air, int64_t B, const int64_t *wall_ids,
+                      int64_t S, int64_t D, const double *brdf,
+                      const int64_t *brdf_index, const double *in_dirs,
+                      double *energy0, double *e0dir, double *distance,
+                      void *stream);
+
+/* ---------------------------------------------------------------------
+ * Energy exchange (the hot loop)
+ *
+ * The exchange runs on a destination-major "plan": for every destination
+ * patch j, the list of source patches i that see it (ascending i, the
+ * reference's accumulation order), each with its delay int(d_ij/c/dt),
+ * outgoing direction p2o[i,j] and coefficient row fft[i,j,:,b].
+ * Plan arrays (caller-owned): row_ptr[P+1] (int64), src[Vd], lag[Vd],
+ * dir[Vd] (int32), coef[Vd,B,DP] (float64, DP = spr_padded_directions(D),
+ * zero-padded).  Vd = 2V.
+ * State (caller-owned, float32, [P,D,B,T] each): prev, cur (order
+ * histograms) and tot_hi, tot_lo (running total as an unevaluated sum
+ * hi + lo, i.e. compensated accumulation).
+ * ------------------------------------------------------------------- */
+
+/* Directions padded to the kernel's register tile (1,2,4,...,64); 0 when
+ * D is unsupported (D > 64). */
+int64_t spr_padded_directions(int64_t D);
+
+/* Plan structure from the visibility matrix: row_ptr and src.
+ * Pass 1 (src == NULL) writes row_ptr (row_ptr[P] = Vd); pass 2 fills src.
+ * The rows list, for destination j, every i != j with vis[min(i,j),
+ * max(i,j)] = 1 in ascending order. */
+int spr_plan_rows(int64_t P, const uint8_t *vis, int64_t *row_ptr,
+                  int32_t *src, void *stream);
+
+/* Plan delays, directions and coefficients from the reference's dense
+ * arrays (the inputs of _energy_exchange, RadiosityFast.py:1073-1076):
+ * dist[P,P], p2o[P,P] (int64), fft[P,P,D,B] (float64). */
+int spr_plan_from_dense(int64_t P, int64_t D, int64_t B, double c, double dt,
+                        int64_t Vd, const int64_t *row_ptr,
+                        const int32_t *src, const double *dist,
+                        const int64_t *p2o, const double *fft, int32_t *lag,
+                        int32_t *dir, double *coef, void *stream);
+
+/* Plan delays, directions and coefficients computed directly from the
+ * geometry (fuses RadiosityFast.py:538-543 pair distances, the p2o map and
+ * _form_factors_with_directivity_dim without materialising the dense
+ * [P,P,D,B] tensor).  Arguments as spr_fft_dense plus p2o[P,P]; the
+ * coefficients are bit-identical to spr_fft_dense's values. */
+int spr_plan_from_geometry(int64_t P, int64_t D, int64_t B, double c,
+                           double dt, int64_t Vd, const int64_t *row_ptr,
+                           const int32_t *src, const double *centers,
+                           const double *areas, const double *ff,
+                           const int64_t *p2o, const double *air,
+                           const int64_t *wall_ids, int64_t S,
+                           const double *brdf, const int64_t *brdf_index,
+                           const double *in_dirs, int32_t *lag, int32_t *dir,
+                           double *coef, void *stream);
+
+/* _energy_exchange_init_energy (RadiosityFast.py:1037-1070): order 0.
+ * e0dir[P,D,B] (float64), d0[P] -> prev = tot_hi = histogram with e0dir at
+ * bin int(d0/c/dt), tot_lo = residual.  Delays >= T set *err_flag to 1
+ * (device int, caller clears it) and are skipped. */
+int spr_exchange_init(int64_t P, int64_t D, int64_t B, int64_t T,
+                      const double *e0dir, const double *d0, double c,
+                      double dt, float *prev, float *tot_hi, float *tot_lo,
+                      int *err_flag, void *stream);
+
+/* One reflection order of _energy_exchange (RadiosityFast.py:1121-1144):
+ * cur[j,:,b,t] = sum_i coef[e,b,:] * prev[i, dir[e], b, t - lag[e]]
+ * (per-order sums accumulated in float64), then tot += cur. */
+int spr_exchange_step(int64_t P, int64_t D, int64_t B, int64_t T,
+                      const int64_t *row_ptr, const int32_t *src,
+                      const int32_t *lag, const int32_t *dir,
+                      const double *coef, const float *prev, float *cur,
+                      float *tot_hi, float *tot_lo, void *stream);
+
+/* etc[P,D,B,T] (float32) = tot_hi + tot_lo (may alias tot_hi). */
+int spr_exchange_finalize(int64_t n, const float *tot_hi,
+                          const float *tot_lo, float *etc, void *stream);
+
+/* Bytes of workspace spr_energy_exchange needs. */
+int64_t spr_energy_exchange_workspace(int64_t P, int64_t D, int64_t B,
+                                      int64_t T, int64_t V);
+
+/* _energy_exchange (RadiosityFast.py:1073-1145) with the reference's
+ * argument list: T, e0dir[P,D,B], d0[P], dist[P,P], fft[P,P,D,B],
+ * p2o[P,P] (int64), c, dt, K, pairs[V,2] (int32) -> etc[P,D,B,T]
+ * (float32).  workspace: spr_energy_exchange_workspace(...) bytes.
+ * Synchronous: returns SPR_ERR_RANGE if a source delay is >= T (the
+ * reference raises IndexError there). */
+int spr_energy_exchange(int64_t T, int64_t P, int64_t D, int64_t B,
+                        const double *e0dir, const double *d0,
+                        const double *dist, const double *fft,
+                        const int64_t *p2o, double c, double dt, int64_t K,
+                        int64_t V, const int32_t *pairs, float *etc,
+                        void *workspace, int64_t workspace_bytes,
+                        void *stream);
+
+/* ---------------------------------------------------------------------
+ * Receiver collection
+ * ------------------------------------------------------------------- */
+
+/* _patch2receiver_energy_universal (universal.py:149-160), batched:
+ * points[NQ,3], patch points[P,4,3], vis[NQ,P] -> factor[NQ,P] (float64). */
+int spr_receiver_energy(int64_t NQ, const double *points, int64_t P,
+                        const double *patch_points, const uint8_t *vis,
+                        double *factor, void *stream);
+
+/* _collect_energy_patches + _collect_receiver_energy
+ * (RadiosityFast.py:686-752, 1148-1185) and the patch sum of
+ * collect_energy_receiver_mono (RadiosityFast.py:591-592):
+ * out[r,(k,)b,t] = sum_k etc[k, idx[r,k], b, (t - n_rk) mod T] *
+ *                  factor[r,k] * exp(-air_b * d_rk),
+ * n_rk = ceil(d_rk/c/dt), d_rk = |c_k - point_r|.
+ * etc[P,D,B,T] (float32), points[NR,3], centers[P,3], idx[NR,P] (int64),
+ * factor[NR,P], air[B].  patchwise = 0: out[NR,B,T] (float32, patch sum in
+ * float64); patchwise = 1: out[NR,P,B,T] (float32). */
+int spr_collect(int64_t NR, int64_t P, int64_t D, int64_t B, int64_t T,
+                const float *etc, const double *points,
+                const double *centers, const int64_t *idx,
+                const double *factor, const double *air, double c, double dt,
+                int patchwise, float *out, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

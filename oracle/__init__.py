@@ -1,0 +1,292 @@
+"""FP64 CPU oracle of sparrowpy's DirectionalRadiosityFast hot path.
+
+TEST INFRASTRUCTURE ONLY.  Imported by ``tests/``, the package smoke check and
+the benchmark's CPU-baseline leg as the checker.  The product package
+(``sparrowpy_amd``) never imports it.
+
+``oracle.c_*`` wrap the C restatement in ``sparrow_oracle.c`` (one function
+per reference function, see its header for file:line citations);
+``oracle.pipeline.OracleRadiosity`` composes them exactly as the reference
+class methods do.
+
+Pinning (what checks this oracle against something other than itself):
+see ``tests/test_oracle_pins.py``.  Every function below is pinned by a
+closed form, a reference known-answer test or an invariant, except the
+ULP-level choices documented in ``sparrow_oracle.c`` (BLAS/LAPACK-dependent
+dot/norm/inv rounding), which no available check can pin.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+_f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_u8p = np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS")
+_i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_i64p = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
+_i64 = ctypes.c_int64
+_dbl = ctypes.c_double
+_int = ctypes.c_int
+
+
+def build(force=False):
+    """Compile liboracle.so with the Makefile next to this file."""
+    if force or not os.path.exists(_LIB_PATH):
+        subprocess.run(["make", "-s", "-C", _HERE, "-B" if force else "all"],
+                       check=True)
+
+
+def _load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(_LIB_PATH):
+        build()
+    lib = ctypes.CDLL(_LIB_PATH)
+    sig = {
+        "orc_basic_visibility": (_int, [_f64p, _f64p, _f64p, _int, _f64p]),
+        "orc_point_in_polygon": (_int, [_f64p, _f64p, _int, _f64p]),
+        "orc_p2p_visibility": (None, [_i64, _f64p, _i64, _f64p, _f64p, _int,
+                                      _u8p, _int]),
+        "orc_point_visibility": (None, [_f64p, _i64, _f64p, _i64, _f64p,
+                                        _f64p, _int, _u8p]),
+        "orc_stokes": (_dbl, [_f64p, _f64p, _dbl]),
+        "orc_nusselt": (_dbl, [_f64p, _f64p, _f64p, _f64p, _int]),
+        "orc_universal_ff": (_dbl, [_f64p, _f64p, _dbl, _f64p, _f64p]),
+        "orc_form_factors": (None, [_i64, _f64p, _f64p, _f64p, _i64, _i32p,
+                                    _f64p, _int]),
+        "orc_pt_solution": (_dbl, [_f64p, _f64p, _int, _int]),
+        "orc_source_energy": (None, [_f64p, _i64, _f64p, _f64p, _u8p,
+                                     ctypes.c_void_p, _i64, _f64p, _f64p]),
+        "orc_receiver_energy": (None, [_f64p, _i64, _f64p, _u8p, _f64p]),
+        "orc_receiver_index": (None, [_i64, _f64p, _f64p, _i64, _f64p, _i64p,
+                                      _i64p]),
+        "orc_source_index": (_i64, [_f64p, _f64p, _i64, _f64p, _i64]),
+        "orc_fft": (None, [_i64, _u8p, _f64p, _i64, _f64p, _f64p,
+                           ctypes.c_void_p, _i64p, _i64, _i64,
+                           ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                           _f64p, _int]),
+        "orc_add_directional": (None, [_i64, _i64, _f64p, _f64p, _f64p,
+                                       _i64p, _i64, _i64, _f64p, _f64p,
+                                       _i64p, _f64p]),
+        "orc_init_energy": (_int, [_i64, _i64, _i64, _i64, _f64p, _f64p,
+                                   _dbl, _dbl, _f64p]),
+        "orc_exchange": (_int, [_i64, _i64, _i64, _i64, _f64p, _f64p, _f64p,
+                                _f64p, _i64p, _dbl, _dbl, _i64, _i64, _i32p,
+                                _f64p, _f64p, _int]),
+        "orc_exchange_order": (None, [_i64, _i64, _i64, _i64, _f64p, _f64p,
+                                      _i64p, _i32p, _i32p, _i32p, _f64p,
+                                      _i64, _i64, _int]),
+        "orc_collect_receiver_energy": (None, [_i64, _i64, _i64, _f64p,
+                                               _f64p, _dbl, _dbl, _f64p,
+                                               _f64p]),
+        "orc_delay_floor": (_i64, [_dbl, _dbl, _dbl]),
+        "orc_delay_ceil": (_i64, [_dbl, _dbl, _dbl]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _c(a, dtype=np.float64):
+    return np.ascontiguousarray(a, dtype=dtype)
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def default_threads():
+    return int(os.environ.get("ORACLE_THREADS", os.cpu_count() or 1))
+
+
+# -- geometry (geometry.py) --------------------------------------------
+def basic_visibility(vis_point, eval_point, surf_points, surf_normal):
+    sp = _c(surf_points)
+    return bool(_load().orc_basic_visibility(
+        _c(vis_point), _c(eval_point), sp, sp.shape[0], _c(surf_normal)))
+
+
+def point_in_polygon(point, polygon, normal):
+    pg = _c(polygon)
+    return bool(_load().orc_point_in_polygon(_c(point), pg, pg.shape[0],
+                                             _c(normal)))
+
+
+def p2p_visibility(centers, surf_normals, surf_points, nthreads=None):
+    """geometry.py:750-797 ``_check_patch2patch_visibility``."""
+    centers = _c(centers)
+    sp = _c(surf_points)
+    P = centers.shape[0]
+    out = np.zeros((P, P), dtype=np.uint8)
+    _load().orc_p2p_visibility(P, centers, sp.shape[0], _c(surf_normals), sp,
+                               sp.shape[1], out,
+                               nthreads or default_threads())
+    return out.astype(bool)
+
+
+def point_visibility(point, centers, surf_normals, surf_points):
+    """geometry.py:799-839 ``_check_point2patch_visibility``."""
+    centers = _c(centers)
+    sp = _c(surf_points)
+    out = np.zeros(centers.shape[0], dtype=np.uint8)
+    _load().orc_point_visibility(_c(point), centers.shape[0], centers,
+                                 sp.shape[0], _c(surf_normals), sp,
+                                 sp.shape[1], out)
+    return out.astype(bool)
+
+
+# -- form factors (universal.py, integration.py) -------------------------
+def stokes(patch_i, patch_j, area_i):
+    return _load().orc_stokes(_c(patch_i), _c(patch_j), float(area_i))
+
+
+def nusselt(patch_i, patch_j, normal_i, normal_j, nsamples=64):
+    return _load().orc_nusselt(_c(patch_i), _c(patch_j), _c(normal_i),
+                               _c(normal_j), nsamples)
+
+
+def universal_ff(src_pts, src_normal, src_area, rcv_pts, rcv_normal):
+    return _load().orc_universal_ff(_c(src_pts), _c(src_normal),
+                                    float(src_area), _c(rcv_pts),
+                                    _c(rcv_normal))
+
+
+def form_factors(pts, normals, areas, pairs, nthreads=None):
+    """universal.py:12-52 ``patch2patch_ff_universal``."""
+    pts = _c(pts)
+    P = pts.shape[0]
+    pairs = _c(pairs, np.int32).reshape(-1, 2)
+    out = np.zeros((P, P))
+    _load().orc_form_factors(P, pts, _c(normals), _c(areas), pairs.shape[0],
+                             pairs, out, nthreads or default_threads())
+    return out
+
+
+def pt_solution(point, patch_points, mode="source"):
+    pp = _c(patch_points)
+    return _load().orc_pt_solution(_c(point), pp, pp.shape[0],
+                                   1 if mode == "receiver" else 0)
+
+
+def source_energy(source, centers, pts, vis, air, n_bins):
+    centers = _c(centers)
+    P = centers.shape[0]
+    energy = np.zeros((P, n_bins))
+    dist = np.zeros(P)
+    air_c = None if air is None else _c(air)
+    _load().orc_source_energy(_c(source), P, centers, _c(pts),
+                              _c(vis, np.uint8), _ptr(air_c), n_bins, energy,
+                              dist)
+    return energy, dist
+
+
+def receiver_energy(receiver, pts, vis):
+    pts = _c(pts)
+    out = np.zeros(pts.shape[0])
+    _load().orc_receiver_energy(_c(receiver), pts.shape[0], pts,
+                                _c(vis, np.uint8), out)
+    return out
+
+
+def receiver_index(pos_i, pos_j, receivers, wall_ids):
+    pos_i = _c(pos_i).reshape(-1, 3)
+    rec = _c(receivers)
+    out = np.zeros(pos_i.shape[0], dtype=np.int64)
+    _load().orc_receiver_index(pos_i.shape[0], pos_i, _c(pos_j), rec.shape[1],
+                               rec, _c(wall_ids, np.int64), out)
+    return out
+
+
+def source_index(pos_h, pos_i, sources, wall):
+    src = _c(sources)
+    return int(_load().orc_source_index(_c(pos_h), _c(pos_i), src.shape[1],
+                                        src, int(wall)))
+
+
+def fft(vis, ff, n_bins, centers, areas, air, wall_ids, brdf, brdf_index,
+        sources, receivers, nthreads=None):
+    """RadiosityFast.py:1234-1272 ``_form_factors_with_directivity_dim``."""
+    vis = _c(vis, np.uint8)
+    P = vis.shape[0]
+    if brdf is None:
+        S, D = 1, 1
+        brdf_c = idx_c = src_c = None
+    else:
+        brdf_c = _c(brdf)
+        S, D = brdf_c.shape[1], np.asarray(receivers).shape[1]
+        idx_c = _c(brdf_index, np.int64)
+        src_c = _c(sources)
+    out = np.zeros((P, P, D, n_bins))
+    air_c = None if air is None else _c(air)
+    _load().orc_fft(P, vis, _c(ff), n_bins, _c(centers), _c(areas),
+                    _ptr(air_c), _c(wall_ids, np.int64), S, D, _ptr(brdf_c),
+                    _ptr(idx_c), _ptr(src_c), out,
+                    nthreads or default_threads())
+    return out
+
+
+def add_directional(energy0, source, centers, wall_ids, sources, receivers,
+                    brdf, brdf_index):
+    energy0 = _c(energy0)
+    P, B = energy0.shape
+    src = _c(sources)
+    brdf = _c(brdf)
+    D = np.asarray(receivers).shape[1]
+    out = np.zeros((P, D, B))
+    _load().orc_add_directional(P, B, energy0, _c(source), _c(centers),
+                                _c(wall_ids, np.int64), src.shape[1], D, src,
+                                brdf, _c(brdf_index, np.int64), out)
+    return out
+
+
+def init_energy(n_samples, e0dir, d0, c, dt):
+    e0dir = _c(e0dir)
+    P, D, B = e0dir.shape
+    out = np.zeros((P, D, B, n_samples))
+    if _load().orc_init_energy(n_samples, P, D, B, e0dir, _c(d0), c, dt,
+                               out):
+        raise IndexError("source delay exceeds the histogram length")
+    return out
+
+
+def exchange(n_samples, e0dir, d0, dist, fft_, p2o, c, dt, max_order, pairs,
+             nthreads=1):
+    """RadiosityFast.py:1073-1145 ``_energy_exchange`` (FP64)."""
+    e0dir = _c(e0dir)
+    P, D, B = e0dir.shape
+    pairs = _c(pairs, np.int32).reshape(-1, 2)
+    out = np.zeros((P, D, B, n_samples))
+    work = np.zeros(2 * out.size)
+    if _load().orc_exchange(n_samples, P, D, B, e0dir, _c(d0), _c(dist),
+                            _c(fft_), _c(p2o, np.int64), c, dt, max_order,
+                            pairs.shape[0], pairs, out, work, nthreads):
+        raise IndexError("source delay exceeds the histogram length")
+    return out
+
+
+def collect_receiver_energy(E, dist, c, dt, air):
+    E = _c(E)
+    P, B, T = E.shape
+    out = np.zeros_like(E)
+    _load().orc_collect_receiver_energy(P, B, T, E, _c(dist), c, dt,
+                                        _c(air), out)
+    return out
+
+
+def delay_floor(d, c, dt):
+    return int(_load().orc_delay_floor(float(d), float(c), float(dt)))
+
+
+def delay_ceil(d, c, dt):
+    return int(_load().orc_delay_ceil(float(d), float(c), float(dt)))

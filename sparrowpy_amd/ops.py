@@ -1,0 +1,404 @@
+"""Operator layer: one function per kernel function of the reference path.
+
+Each function takes numpy arrays or device tensors, runs the HIP kernel on
+the current stream and returns device tensors.  Names and argument meaning
+follow the reference's module-level functions (file:line in each
+docstring); see :mod:`sparrowpy_amd.radiosity` for the class that chains
+them like ``DirectionalRadiosityFast``.
+"""
+from __future__ import annotations
+
+import dataclasses
+
+import numpy as np
+import torch
+
+from . import _hip
+from ._hip import call, ptr, stream_ptr
+
+__all__ = [
+    "device", "as_dev", "check_patch2patch_visibility",
+    "check_point2patch_visibility", "visible_pairs",
+    "patch2patch_ff_universal", "direction_map", "point_direction_index",
+    "form_factors_with_directivity_dim", "source_energy",
+    "receiver_energy", "collect", "energy_exchange", "ExchangePlan",
+    "ExchangeState",
+]
+
+
+def device():
+    _hip.lib()  # raises HipUnavailable without a GPU or library
+    return torch.device("cuda", torch.cuda.current_device())
+
+
+def as_dev(x, dtype):
+    """Contiguous device tensor of ``dtype`` (copies only if needed)."""
+    if x is None:
+        return None
+    if isinstance(x, torch.Tensor):
+        t = x.to(device=device(), dtype=dtype)
+    else:
+        t = torch.as_tensor(np.ascontiguousarray(x), dtype=dtype,
+                            device=device())
+    return t.contiguous()
+
+
+def _f64(x):
+    return as_dev(x, torch.float64)
+
+
+def _i64(x):
+    return as_dev(x, torch.int64)
+
+
+def _u8(x):
+    return as_dev(x, torch.uint8)
+
+
+def _vis_workspace(ns):
+    n = int(_hip.lib().spr_visibility_workspace(int(ns)))
+    return torch.empty(n, dtype=torch.uint8, device=device())
+
+
+def check_patch2patch_visibility(patches_center, surf_normal, surf_points):
+    """geometry.py:750-797 ``_check_patch2patch_visibility`` -> uint8 [P,P]
+    (upper triangle)."""
+    c = _f64(patches_center)
+    n = _f64(surf_normal)
+    p = _f64(surf_points)
+    P, NS = c.shape[0], n.shape[0]
+    if p.shape[1:] != (4, 3):
+        raise ValueError("surfaces must have 4 corners")
+    vis = torch.empty((P, P), dtype=torch.uint8, device=c.device)
+    ws = _vis_workspace(NS)
+    call("spr_p2p_visibility", P, ptr(c), NS, ptr(n), ptr(p), ptr(vis),
+         ptr(ws), ws.numel(), stream_ptr())
+    return vis
+
+
+def check_point2patch_visibility(eval_points, patches_center, surf_normal,
+                                 surf_points):
+    """geometry.py:799-839 ``_check_point2patch_visibility``, batched over
+    query points [NQ,3] -> uint8 [NQ,P]."""
+    q = _f64(np.atleast_2d(eval_points) if not isinstance(
+        eval_points, torch.Tensor) else eval_points.reshape(-1, 3))
+    c = _f64(patches_center)
+    n = _f64(surf_normal)
+    p = _f64(surf_points)
+    NQ, P, NS = q.shape[0], c.shape[0], n.shape[0]
+    vis = torch.empty((NQ, P), dtype=torch.uint8, device=c.device)
+    ws = _vis_workspace(NS)
+    call("spr_point_visibility", NQ, ptr(q), P, ptr(c), NS, ptr(n), ptr(p),
+         ptr(vis), ptr(ws), ws.numel(), stream_ptr())
+    return vis
+
+
+def visible_pairs(vis):
+    """RadiosityFast.py:377-387 -> int32 [V,2], row-major i<j."""
+    vis = _u8(vis)
+    P = vis.shape[0]
+    offs = torch.empty(P + 1, dtype=torch.int64, device=vis.device)
+    call("spr_visible_pairs", P, ptr(vis), ptr(offs), None, stream_ptr())
+    V = int(offs[P].item())
+    pairs = torch.empty((V, 2), dtype=torch.int32, device=vis.device)
+    if V:
+        call("spr_visible_pairs", P, ptr(vis), ptr(offs), ptr(pairs),
+             stream_ptr())
+    return pairs
+
+
+def patch2patch_ff_universal(patches_points, patches_normals, patches_areas,
+                             visible_patches):
+    """universal.py:12-52 ``patch2patch_ff_universal`` -> float64 [P,P]."""
+    pts = _f64(patches_points)
+    nrm = _f64(patches_normals)
+    area = _f64(patches_areas)
+    pairs = as_dev(visible_patches, torch.int32).reshape(-1, 2)
+    P = pts.shape[0]
+    ff = torch.empty((P, P), dtype=torch.float64, device=pts.device)
+    call("spr_form_factors", P, ptr(pts), ptr(nrm), ptr(area),
+         pairs.shape[0], ptr(pairs), ptr(ff), stream_ptr())
+    return ff
+
+
+def direction_map(vis, patches_center, out_dirs, wall_ids):
+    """RadiosityFast.py:403-414 (``get_scattering_data_receiver_index``
+    over visible pairs) -> int64 [P,P], sentinel D."""
+    vis = _u8(vis)
+    c = _f64(patches_center)
+    od = _f64(out_dirs)
+    w = _i64(wall_ids)
+    P, D = c.shape[0], od.shape[1]
+    p2o = torch.empty((P, P), dtype=torch.int64, device=c.device)
+    call("spr_direction_map", P, ptr(vis), ptr(c), D, ptr(od), ptr(w),
+         ptr(p2o), stream_ptr())
+    return p2o
+
+
+def point_direction_index(points, patches_center, out_dirs, wall_ids):
+    """``get_scattering_data_receiver_index(centers, point, ...)``
+    (RadiosityFast.py:728-730) for each point -> int64 [NQ,P]."""
+    q = _f64(points).reshape(-1, 3)
+    c = _f64(patches_center)
+    od = _f64(out_dirs)
+    w = _i64(wall_ids)
+    NQ, P, D = q.shape[0], c.shape[0], od.shape[1]
+    idx = torch.empty((NQ, P), dtype=torch.int64, device=c.device)
+    call("spr_point_direction_index", NQ, ptr(q), P, ptr(c), D, ptr(od),
+         ptr(w), ptr(idx), stream_ptr())
+    return idx
+
+
+def form_factors_with_directivity_dim(visibility_matrix, form_factors,
+                                      n_bins, patches_center, patches_area,
+                                      air_attenuation, patch_to_wall_ids,
+                                      scattering, scattering_index, sources,
+                                      receivers):
+    """RadiosityFast.py:1234-1272 -> float64 [P,P,D,B] (dense)."""
+    vis = _u8(visibility_matrix)
+    ff = _f64(form_factors)
+    c = _f64(patches_center)
+    area = _f64(patches_area)
+    air = _f64(air_attenuation)
+    w = _i64(patch_to_wall_ids)
+    P = c.shape[0]
+    if scattering is None:
+        S, D = 1, 1
+        brdf = bidx = src = None
+    else:
+        brdf = _f64(scattering)
+        bidx = _i64(scattering_index)
+        src = _f64(sources)
+        S, D = src.shape[1], int(np.shape(receivers)[1])
+    out = torch.empty((P, P, D, n_bins), dtype=torch.float64, device=c.device)
+    call("spr_fft_dense", P, ptr(vis), ptr(ff), n_bins, ptr(c), ptr(area),
+         ptr(air), ptr(w), S, D, ptr(brdf), ptr(bidx), ptr(src), ptr(out),
+         stream_ptr())
+    return out
+
+
+def source_energy(source_position, patches_center, patches_points,
+                  source_visibility, air_attenuation, n_bins,
+                  patch_to_wall_ids, sources, scattering, scattering_index,
+                  n_directions):
+    """universal.py:98-147 + RadiosityFast.py:988-1034.
+
+    Returns (energy_0 [P,B], energy_0_dir [P,D,B], distance_0 [P]), float64.
+    """
+    s = _f64(np.asarray(source_position, float).reshape(3)
+             if not isinstance(source_position, torch.Tensor)
+             else source_position.reshape(3))
+    c = _f64(patches_center)
+    pts = _f64(patches_points)
+    vis = _u8(source_visibility)
+    air = _f64(air_attenuation)
+    w = _i64(patch_to_wall_ids)
+    src = _f64(sources)
+    brdf = _f64(scattering)
+    bidx = _i64(scattering_index)
+    P, S, D = c.shape[0], src.shape[1], int(n_directions)
+    e0 = torch.empty((P, n_bins), dtype=torch.float64, device=c.device)
+    e0d = torch.empty((P, D, n_bins), dtype=torch.float64, device=c.device)
+    d0 = torch.empty(P, dtype=torch.float64, device=c.device)
+    call("spr_source_energy", ptr(s), P, ptr(c), ptr(pts), ptr(vis),
+         ptr(air), n_bins, ptr(w), S, D, ptr(brdf), ptr(bidx), ptr(src),
+         ptr(e0), ptr(e0d), ptr(d0), stream_ptr())
+    return e0, e0d, d0
+
+
+def receiver_energy(points, patches_points, receiver_visibility):
+    """universal.py:149-160, batched over points -> float64 [NQ,P]."""
+    q = _f64(points).reshape(-1, 3)
+    pts = _f64(patches_points)
+    vis = _u8(receiver_visibility).reshape(q.shape[0], -1)
+    NQ, P = q.shape[0], pts.shape[0]
+    out = torch.empty((NQ, P), dtype=torch.float64, device=pts.device)
+    call("spr_receiver_energy", NQ, ptr(q), P, ptr(pts), ptr(vis), ptr(out),
+         stream_ptr())
+    return out
+
+
+def collect(etc, points, patches_center, idx, factor, air_attenuation,
+            speed_of_sound, time_resolution, patchwise=False):
+    """RadiosityFast.py:686-752, 1148-1185 (and the patch sum at :592).
+
+    etc float32 [P,D,B,T] -> float32 [NR,B,T] or [NR,P,B,T] (patchwise).
+    """
+    etc = as_dev(etc, torch.float32)
+    q = _f64(points).reshape(-1, 3)
+    c = _f64(patches_center)
+    idx = _i64(idx)
+    g = _f64(factor)
+    air = _f64(air_attenuation)
+    P, D, B, T = etc.shape
+    NR = q.shape[0]
+    shape = (NR, P, B, T) if patchwise else (NR, B, T)
+    out = torch.empty(shape, dtype=torch.float32, device=etc.device)
+    call("spr_collect", NR, P, D, B, T, ptr(etc), ptr(q), ptr(c), ptr(idx),
+         ptr(g), ptr(air), float(speed_of_sound), float(time_resolution),
+         1 if patchwise else 0, ptr(out), stream_ptr())
+    return out
+
+
+def energy_exchange(n_samples, energy_0_directivity, distance_0,
+                    distance_ij, form_factors_tilde, patch_2_out_directions,
+                    speed_of_sound, histogram_time_resolution, max_order,
+                    visible_patches):
+    """RadiosityFast.py:1073-1145 ``_energy_exchange`` with the reference's
+    arguments -> float32 [P,D,B,T]."""
+    e0d = _f64(energy_0_directivity)
+    d0 = _f64(distance_0)
+    dist = _f64(distance_ij)
+    fft = _f64(form_factors_tilde)
+    p2o = _i64(patch_2_out_directions)
+    pairs = as_dev(visible_patches, torch.int32).reshape(-1, 2)
+    P, D, B = e0d.shape
+    T = int(n_samples)
+    V = pairs.shape[0]
+    etc = torch.empty((P, D, B, T), dtype=torch.float32, device=e0d.device)
+    nws = int(_hip.lib().spr_energy_exchange_workspace(P, D, B, T, V))
+    ws = torch.empty(nws, dtype=torch.uint8, device=e0d.device)
+    call("spr_energy_exchange", T, P, D, B, ptr(e0d), ptr(d0), ptr(dist),
+         ptr(fft), ptr(p2o), float(speed_of_sound),
+         float(histogram_time_resolution), int(max_order), V, ptr(pairs),
+         ptr(etc), ptr(ws), nws, stream_ptr())
+    return etc
+
+
+@dataclasses.dataclass
+class ExchangePlan:
+    """Destination-major pair table of one scene (see sparrow_hip.h)."""
+
+    P: int
+    D: int
+    B: int
+    row_ptr: torch.Tensor
+    src: torch.Tensor
+    lag: torch.Tensor
+    dir: torch.Tensor
+    coef: torch.Tensor
+
+    @property
+    def Vd(self):
+        return int(self.src.numel())
+
+    @classmethod
+    def rows(cls, vis):
+        vis = _u8(vis)
+        P = vis.shape[0]
+        row_ptr = torch.empty(P + 1, dtype=torch.int64, device=vis.device)
+        call("spr_plan_rows", P, ptr(vis), ptr(row_ptr), None, stream_ptr())
+        Vd = int(row_ptr[P].item())
+        src = torch.empty(max(Vd, 1), dtype=torch.int32, device=vis.device)
+        if Vd:
+            call("spr_plan_rows", P, ptr(vis), ptr(row_ptr), ptr(src),
+                 stream_ptr())
+        return row_ptr, src[:Vd]
+
+    @classmethod
+    def _alloc(cls, P, D, B, row_ptr, src):
+        DP = int(_hip.lib().spr_padded_directions(D))
+        if DP == 0:
+            raise ValueError(f"{D} directions unsupported (max 64)")
+        Vd = src.numel()
+        dev = src.device
+        lag = torch.empty(max(Vd, 1), dtype=torch.int32, device=dev)[:Vd]
+        dirs = torch.empty(max(Vd, 1), dtype=torch.int32, device=dev)[:Vd]
+        coef = torch.empty((max(Vd, 1), B, DP), dtype=torch.float64,
+                           device=dev)[:Vd]
+        return cls(P, D, B, row_ptr, src, lag, dirs, coef)
+
+    @classmethod
+    def from_dense(cls, vis, distance_ij, patch_2_out_directions,
+                   form_factors_tilde, speed_of_sound, time_resolution):
+        fft = _f64(form_factors_tilde)
+        P, _, D, B = fft.shape
+        row_ptr, src = cls.rows(vis)
+        plan = cls._alloc(P, D, B, row_ptr, src)
+        dist = _f64(distance_ij)
+        p2o = _i64(patch_2_out_directions)
+        call("spr_plan_from_dense", P, D, B, float(speed_of_sound),
+             float(time_resolution), plan.Vd, ptr(row_ptr), ptr(src),
+             ptr(dist), ptr(p2o), ptr(fft), ptr(plan.lag), ptr(plan.dir),
+             ptr(plan.coef), stream_ptr())
+        return plan
+
+    @classmethod
+    def from_geometry(cls, vis, patches_center, patches_area, form_factors,
+                      p2o, air_attenuation, patch_to_wall_ids, sources,
+                      scattering, scattering_index, n_directions, n_bins,
+                      speed_of_sound, time_resolution):
+        c = _f64(patches_center)
+        P = c.shape[0]
+        D, B = int(n_directions), int(n_bins)
+        row_ptr, src = cls.rows(vis)
+        plan = cls._alloc(P, D, B, row_ptr, src)
+        area = _f64(patches_area)
+        ff = _f64(form_factors)
+        p2o = _i64(p2o)
+        air = _f64(air_attenuation)
+        w = _i64(patch_to_wall_ids)
+        if scattering is None:
+            S, brdf, bidx, sd = 1, None, None, None
+        else:
+            sd = _f64(sources)
+            S = sd.shape[1]
+            brdf = _f64(scattering)
+            bidx = _i64(scattering_index)
+        call("spr_plan_from_geometry", P, D, B, float(speed_of_sound),
+             float(time_resolution), plan.Vd, ptr(row_ptr), ptr(src), ptr(c),
+             ptr(area), ptr(ff), ptr(p2o), ptr(air), ptr(w), S, ptr(brdf),
+             ptr(bidx), ptr(sd), ptr(plan.lag), ptr(plan.dir),
+             ptr(plan.coef), stream_ptr())
+        return plan
+
+
+class ExchangeState:
+    """Order histograms and the compensated running total of one source.
+
+    ``prev``/``cur`` are float32 [P,D,B,T]; the total is ``hi + lo``.
+    """
+
+    def __init__(self, P, D, B, T, dev=None):
+        dev = dev or device()
+        shape = (P, D, B, T)
+        self.shape = shape
+        self.prev = torch.empty(shape, dtype=torch.float32, device=dev)
+        self.cur = torch.empty(shape, dtype=torch.float32, device=dev)
+        self.hi = torch.empty(shape, dtype=torch.float32, device=dev)
+        self.lo = torch.empty(shape, dtype=torch.float32, device=dev)
+        self.err = torch.zeros(1, dtype=torch.int32, device=dev)
+
+    def init(self, energy_0_directivity, distance_0, speed_of_sound,
+             time_resolution):
+        """Order 0 (RadiosityFast.py:1037-1070)."""
+        e0d = _f64(energy_0_directivity)
+        d0 = _f64(distance_0)
+        P, D, B, T = self.shape
+        self.err.zero_()
+        call("spr_exchange_init", P, D, B, T, ptr(e0d), ptr(d0),
+             float(speed_of_sound), float(time_resolution), ptr(self.prev),
+             ptr(self.hi), ptr(self.lo), ptr(self.err), stream_ptr())
+
+    def step(self, plan: ExchangePlan):
+        """One reflection order; swaps prev and cur."""
+        P, D, B, T = self.shape
+        call("spr_exchange_step", P, D, B, T, ptr(plan.row_ptr),
+             ptr(plan.src), ptr(plan.lag), ptr(plan.dir), ptr(plan.coef),
+             ptr(self.prev), ptr(self.cur), ptr(self.hi), ptr(self.lo),
+             stream_ptr())
+        self.prev, self.cur = self.cur, self.prev
+
+    def finalize(self, out=None):
+        """float32 total (hi + lo); ``out`` may be ``self.hi``."""
+        out = torch.empty_like(self.hi) if out is None else out
+        call("spr_exchange_finalize", self.hi.numel(), ptr(self.hi),
+             ptr(self.lo), ptr(out), stream_ptr())
+        return out
+
+    def check(self):
+        """Raise IndexError if a source delay fell beyond the histogram
+        (the reference raises IndexError at RadiosityFast.py:1069)."""
+        if int(self.err.item()):
+            raise IndexError(
+                "source-to-patch delay exceeds the histogram length")
