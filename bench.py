@@ -52,53 +52,61 @@ def _cpu_info():
     return model
 
 
+def _generic_rows(plan, D):
+    """Host copies of a plan as per-entry rows: (row_ptr, src, lag, dir,
+    coef[e, d] for band 0) -- the reference's per-pair update."""
+    from sparrowpy_amd import ops
+    row_ptr = plan.row_ptr.cpu().numpy()
+    lag = plan.lag.cpu().numpy()
+    if isinstance(plan, ops.GroupedPlan):
+        rowid = plan.rowid.cpu().numpy().astype(np.int64)
+        src, dirs = rowid // D, rowid % D
+        gs = plan.grp_start.cpu().numpy()
+        gk = plan.grp_key.cpu().numpy()
+        g_of_e = np.searchsorted(gs, np.arange(rowid.size), side="right") - 1
+        rt = plan.rtab[:, 0, :D].cpu().numpy()
+        coef = plan.scal[:, 0].cpu().numpy()[:, None] * rt[gk[g_of_e]]
+    else:
+        src = plan.src.cpu().numpy()
+        dirs = plan.dir.cpu().numpy()
+        coef = plan.coef[:, 0, :D].cpu().numpy()
+    return row_ptr, src, lag, dirs, coef
+
+
 def cpu_baseline(plan, D, T, seconds, seed=0):
     """Time the FP64 oracle's exchange order on a bounded sample.
 
     Sample: a seeded subset of destination patches with their full source
     lists, band 0 only, the histogram truncated to ``T_s`` bins, all host
-    threads the process may use.  Returns (updates/s, threads, T_s, n_dest).
+    threads the process may use.  The oracle performs the reference's
+    per-pair D-wide update.  Returns (updates/s, threads, T_s, n_dest).
     """
     import oracle as orc
-    import torch
     orc.build()
     lib = orc._load()
     nthreads = int(os.environ.get("OMP_NUM_THREADS",
                                   min(os.cpu_count() or 1, 16)))
-    row_ptr = plan.row_ptr.cpu().numpy()
-    src = plan.src.cpu().numpy()
-    lag = plan.lag.cpu().numpy()
-    dirs = plan.dir.cpu().numpy()
+    row_ptr, src, lag, dirs, coef_all = _generic_rows(plan, D)
     P = plan.P
     T_s = min(T, 2400)
     rng = np.random.default_rng(seed)
     prev = np.full((P, D, 1, T_s), 1e-3)
-    cur = np.zeros_like(prev)
     order = rng.permutation(P)
 
     def run(dests):
-        rp = [0]
-        s_, l_, d_, c_ = [], [], [], []
-        for j in dests:
-            a, b = row_ptr[j], row_ptr[j + 1]
-            s_.append(src[a:b])
-            l_.append(lag[a:b])
-            d_.append(dirs[a:b])
-            rp.append(rp[-1] + (b - a))
-        s_ = np.concatenate(s_).astype(np.int32)
-        l_ = np.concatenate(l_).astype(np.int32)
-        d_ = np.concatenate(d_).astype(np.int32)
-        coef = plan.coef[torch.as_tensor(np.concatenate(
-            [np.arange(row_ptr[j], row_ptr[j + 1]) for j in dests]),
-            device=plan.coef.device), 0, :D].double().cpu().numpy()
-        # destinations re-indexed 0..n-1 in a compact cur buffer
-        n = len(dests)
-        cur_s = np.zeros((n, D, 1, T_s))
-        rp = np.asarray(rp, np.int64)
+        idx = np.concatenate([np.arange(row_ptr[j], row_ptr[j + 1])
+                              for j in dests])
+        rp = np.zeros(len(dests) + 1, np.int64)
+        rp[1:] = np.cumsum([row_ptr[j + 1] - row_ptr[j] for j in dests])
+        s_ = np.ascontiguousarray(src[idx], dtype=np.int32)
+        l_ = np.ascontiguousarray(lag[idx], dtype=np.int32)
+        d_ = np.ascontiguousarray(dirs[idx], dtype=np.int32)
+        coef = np.ascontiguousarray(coef_all[idx], dtype=np.float64)
+        cur_s = np.zeros((len(dests), D, 1, T_s))
         upd = int(np.sum(np.maximum(0, T_s - l_.astype(np.int64))))
         t0 = time.perf_counter()
         lib.orc_exchange_order(T_s, P, D, 1, prev, cur_s, rp, s_, l_, d_,
-                               np.ascontiguousarray(coef), 0, n, nthreads)
+                               coef, 0, len(dests), nthreads)
         return upd, time.perf_counter() - t0
 
     # calibrate on a few destinations, then size the sample to `seconds`
@@ -119,6 +127,10 @@ def main(argv=None):
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="target CPU time of the oracle baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--plan", default="auto",
+                    choices=["auto", "grouped", "generic"],
+                    help="exchange plan (auto: grouped when sources share "
+                         "BRDF rows)")
     args = ap.parse_args(argv)
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not "
@@ -139,6 +151,9 @@ def main(argv=None):
         torch.cuda.synchronize()
         return time.perf_counter()
 
+    def log(msg):
+        print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
     t0 = tic()
     r = DirectionalRadiosityFast.from_polygon(sc.walls, sc.patch_size)
     for idx, br in sc.brdf_groups:
@@ -148,13 +163,17 @@ def main(argv=None):
     t0 = tic()
     r.bake_geometry()
     t["bake_s"] = tic() - t0
+    log(f"bake {t['bake_s']:.2f} s, P={r.n_patches}")
     t0 = tic()
     r.init_source_energy(Coordinates(*sc.sources[0]))
     t["inject_s"] = tic() - t0
     c, dt_bin = sc.speed_of_sound, sc.time_resolution
     t0 = tic()
-    plan = r._exchange_plan(c, dt_bin)
+    plan = r._exchange_plan(c, dt_bin,
+                            None if args.plan == "auto" else args.plan)
+    kind = "grouped" if isinstance(plan, ops.GroupedPlan) else "generic"
     t["plan_s"] = tic() - t0
+    log(f"plan {t['plan_s']:.2f} s ({kind}), directed pairs={plan.Vd}")
 
     e0d = r._dev["energy_init_source"]
     d0 = r._dev["distance_patches_to_source"]
@@ -165,6 +184,7 @@ def main(argv=None):
     for _ in range(args.warmup):
         state.step(plan)
     torch.cuda.synchronize()
+    log(f"warmup {args.warmup} orders done")
 
     lag = plan.lag.to(torch.int64)
     U = int((B * torch.clamp(T - lag, min=0)).sum().item())
@@ -181,6 +201,7 @@ def main(argv=None):
     torch.cuda.synchronize()
     wall = time.perf_counter() - wall0
     kern_ms = [s.elapsed_time(e) for s, e in zip(starts, ends)]
+    log(f"timed {args.steps} orders: {wall:.3f} s")
     avg_s = float(np.mean(kern_ms)) / 1e3
     state.check()
 
@@ -196,6 +217,9 @@ def main(argv=None):
 
     value = U * args.steps / wall
     n_state = P * D * B * T
+    # algorithmic bytes per order: gathered source bins, the F~ slice and
+    # delay/direction of each directed pair, one write of the order's
+    # histogram and a read + write of the running total
     alg_bytes = 4 * U + Vd * (4 * D * B + 8) + 12 * n_state
     achieved_gbs = alg_bytes / avg_s / 1e9
     flops = 2 * D * U
@@ -217,12 +241,15 @@ def main(argv=None):
                         f"P={P}, D={D}, B={B}, T={T}, one order per step",
             "patches": P, "directions": D, "bands": B, "bins": T,
             "directed_pairs": Vd, "updates_per_order": U,
+            "plan": kind,
+            "groups": plan.n_groups if kind == "grouped" else None,
         },
         "roofline": {
             "bound": "hbm", "achieved": achieved_gbs, "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": achieved_gbs / HBM_PEAK_GBS,
             "traffic": None,
-            "kernel": "exchange_step_kernel",
+            "kernel": "exchange_grouped_kernel" if kind == "grouped"
+                      else "exchange_step_kernel",
             "kernel_ms": avg_s * 1e3,
             "algorithmic_bytes": alg_bytes,
             "valu_tflops": flops / avg_s / 1e12,
@@ -230,6 +257,7 @@ def main(argv=None):
         "setup": t,
     }
     if not args.no_cpu_baseline:
+        log("cpu baseline sample")
         ups, nth, T_s, n_dest = cpu_baseline(plan, D, T, args.cpu_seconds)
         line["cpu_baseline"] = {
             "value": ups, "unit": "updates/s", "cores": nth, "kind": "port",

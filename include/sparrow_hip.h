@@ -213,6 +213,51 @@ int spr_exchange_step(int64_t P, int64_t D, int64_t B, int64_t T,
                       const double *coef, const float *prev, float *cur,
                       float *tot_hi, float *tot_lo, void *stream);
 
+/* Grouped plan.  The reference builds every coefficient row as
+ * F~[i,j,:,b] = (F~_ij * exp(-air_b |u_ij|)) * brdf[bi(w_i), s_ij, :, b]
+ * (RadiosityFast.py:1255-1270), so sources of one destination that share
+ * the BRDF row (bi(w_i), s_ij) can be summed with their scalar weights first
+ * and spread over the D directions once per group.
+ * Pass 1: src[Vd] (plan rows, in/out) is reordered within each row by the
+ * row key bi(w_i)*S + s_ij (stable), key[Vd] receives the keys and
+ * grp_ptr[P+1] the exclusive scan of the group counts (grp_ptr[P] = G).
+ * max_row = longest row (row_ptr[j+1]-row_ptr[j]); rows up to 8192 entries
+ * are supported (SPR_ERR_UNSUPPORTED otherwise).  in_dirs may be NULL (no
+ * BRDF: one group per row). */
+int spr_plan_grouped_sort(int64_t P, int64_t max_row, const int64_t *row_ptr,
+                          int32_t *src, const double *centers,
+                          const int64_t *wall_ids, int64_t S,
+                          const int64_t *brdf_index, const double *in_dirs,
+                          int32_t *key, int64_t *grp_ptr, void *stream);
+
+/* Pass 2: grp_start[G+1] (int64, first entry of each group, grp_start[G] =
+ * Vd), grp_key[G] (int32), rowid[Vd] (int32, src*D + p2o[src,j]),
+ * lag[Vd] (int32: the delay, with bit 30 set on the last entry of each
+ * group), scal[B,Vd] (float64, band-major, F~_ij * exp(-air_b |u_ij|)) and
+ * rtab[NK,B,DP] (float64, the BRDF rows with NK = NBRDF*S; brdf may be NULL
+ * with D = NK = 1). */
+int spr_plan_grouped_fill(int64_t P, int64_t D, int64_t B, double c,
+                          double dt, int64_t Vd, const int64_t *row_ptr,
+                          const int32_t *src, const int32_t *key,
+                          const int64_t *grp_ptr, const double *centers,
+                          const double *areas, const double *ff,
+                          const int64_t *p2o, const double *air, int64_t NK,
+                          const double *brdf, int64_t *grp_start,
+                          int32_t *grp_key, int32_t *rowid, int32_t *lag,
+                          double *scal, double *rtab, void *stream);
+
+/* One reflection order on a grouped plan (same result as
+ * spr_exchange_step on the equivalent generic plan): a gather pass writes
+ * the per-group sums gsum[G,B,T] (float32 workspace), a spread pass turns
+ * them into cur and the running total. */
+int spr_exchange_step_grouped(int64_t P, int64_t D, int64_t B, int64_t T,
+                              int64_t Vd, const int64_t *row_ptr,
+                              const int64_t *grp_ptr, const int32_t *grp_key,
+                              const int32_t *rowid, const int32_t *lag,
+                              const double *scal, const double *rtab,
+                              float *gsum, const float *prev, float *cur,
+                              float *tot_hi, float *tot_lo, void *stream);
+
 /* etc[P,D,B,T] (float32) = tot_hi + tot_lo (may alias tot_hi). */
 int spr_exchange_finalize(int64_t n, const float *tot_hi,
                           const float *tot_lo, float *etc, void *stream);

@@ -75,6 +75,15 @@ SYMBOLS = {
     "spr_exchange_step": (_int, [_i64, _i64, _i64, _i64, _vp, _vp, _vp, _vp,
                                  _vp, _vp, _vp, _vp, _vp, _vp]),
     "spr_exchange_finalize": (_int, [_i64, _vp, _vp, _vp, _vp]),
+    "spr_plan_grouped_sort": (_int, [_i64, _i64, _vp, _vp, _vp, _vp, _i64,
+                                     _vp, _vp, _vp, _vp, _vp]),
+    "spr_plan_grouped_fill": (_int, [_i64, _i64, _i64, _dbl, _dbl, _i64, _vp,
+                                     _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
+                                     _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
+                                     _vp]),
+    "spr_exchange_step_grouped": (_int, [_i64, _i64, _i64, _i64, _i64, _vp,
+                                         _vp, _vp, _vp, _vp, _vp, _vp, _vp,
+                                         _vp, _vp, _vp, _vp, _vp]),
     "spr_energy_exchange_workspace": (_i64, [_i64, _i64, _i64, _i64, _i64]),
     "spr_energy_exchange": (_int, [_i64, _i64, _i64, _i64, _vp, _vp, _vp,
                                    _vp, _vp, _dbl, _dbl, _i64, _i64, _vp,
@@ -110,6 +119,18 @@ def lib(require_gpu=True):
     return _lib
 
 
+class DevicePointer(ctypes.c_void_p):
+    """A device address that keeps its tensor alive.
+
+    Arguments like ``ptr(as_dev(x))`` create a temporary tensor; holding it
+    here keeps its storage allocated until the entry point has been called
+    (kernels are then enqueued in stream order before any reuse)."""
+
+    def __init__(self, tensor):
+        super().__init__(tensor.data_ptr())
+        self.tensor = tensor
+
+
 def ptr(t):
     """Device pointer of a tensor (None -> NULL)."""
     if t is None:
@@ -118,7 +139,7 @@ def ptr(t):
         raise ValueError("expected a device tensor")
     if not t.is_contiguous():
         raise ValueError("expected a contiguous tensor")
-    return ctypes.c_void_p(t.data_ptr())
+    return DevicePointer(t)
 
 
 def stream_ptr(stream=None):

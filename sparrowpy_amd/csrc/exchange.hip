@@ -159,51 +159,54 @@ __global__ void init_kernel(int64_t P, int64_t D, int64_t B, int64_t T,
 // ---------------------------------------------------------------------
 // one reflection order
 // ---------------------------------------------------------------------
-template <int DP, int R, int BS>
-__global__ __launch_bounds__(BS) void exchange_step_kernel(
-    int D, int B, int T, int n_tblk, int64_t nblocks,
-    const int64_t *__restrict__ row_ptr, const int32_t *__restrict__ src,
-    const int32_t *__restrict__ lag, const int32_t *__restrict__ dir,
-    const double *__restrict__ coef, const float *__restrict__ prev,
-    float *__restrict__ cur, float *__restrict__ tot_hi,
-    float *__restrict__ tot_lo) {
-  const int64_t tile = xcd_tile(blockIdx.x, nblocks);
-  const int tb = (int)(tile % n_tblk);
-  const int64_t jb = tile / n_tblk;
-  const int b = (int)(jb % B);
-  const int64_t j = jb / B;
-  const int t0 = tb * (BS * R);
-  const int t_end = min(T, t0 + BS * R);
-  const int64_t dstride = (int64_t)B * T;  // between directions of a patch
 
-  double acc[DP][R];
-#pragma unroll
-  for (int d = 0; d < DP; ++d)
-#pragma unroll
-    for (int r = 0; r < R; ++r) acc[d][r] = 0.0;
+// Entry metadata of the grouped plan: the delay word carries, in bit 30,
+// the flag "last entry of its group" (delays are clamped below 2^30).
+constexpr int32_t kLastInGroup = 0x40000000;
+constexpr int32_t kLagMask = 0x3fffffff;
 
-  const int64_t e_begin = row_ptr[j], e_end = row_ptr[j + 1];
-  for (int64_t e = e_begin; e < e_end; ++e) {
-    const int n = lag[e];
-    if (n >= t_end) continue;
-    const float *s =
-        prev + ((int64_t)src[e] * D + dir[e]) * dstride + (int64_t)b * T;
-    double ev[R];
+// Gather of C consecutive plan entries starting at e (row ends at e_end):
+// the prev histogram row `row(e)` shifted by the entry's delay.  Metadata
+// loads are unconditional (the index is clamped into the row) so they can
+// be issued together; entries past the row end or whose delay reaches
+// beyond the tile are inactive and load nothing.
+template <int R, int BS, int C, typename RowFn>
+__device__ __forceinline__ void gather_chunk(
+    int64_t e, int64_t e_end, int B, int T, int b, int t0, int t_end,
+    RowFn row_of, const int32_t *__restrict__ lag,
+    const float *__restrict__ prev, float (&buf)[C][R], int (&n)[C],
+    bool (&act)[C]) {
+  const int64_t dstride = (int64_t)B * T;
+  int64_t row[C];
+#pragma unroll
+  for (int c = 0; c < C; ++c) {
+    const int64_t ee = e + c < e_end ? e + c : e_end - 1;
+    n[c] = lag[ee];
+    row[c] = row_of(ee);
+    act[c] = (e + c < e_end) && ((n[c] & kLagMask) < t_end);
+  }
+#pragma unroll
+  for (int c = 0; c < C; ++c) {
+    const float *s = prev + row[c] * dstride + (int64_t)b * T;
+    const int nc = n[c] & kLagMask;
 #pragma unroll
     for (int r = 0; r < R; ++r) {
       const int t = t0 + (int)threadIdx.x + BS * r;
-      const int ts = t - n;
-      ev[r] = (t < T && ts >= 0) ? (double)s[ts] : 0.0;
-    }
-    const double *cf = coef + (e * B + b) * DP;
-#pragma unroll
-    for (int d = 0; d < DP; ++d) {
-      const double cd = cf[d];
-#pragma unroll
-      for (int r = 0; r < R; ++r) acc[d][r] = fma(cd, ev[r], acc[d][r]);
+      const int ts = t - nc;
+      buf[c][r] = (act[c] && t < T && ts >= 0) ? s[ts] : 0.f;
     }
   }
+}
 
+// Write one order's tile (cur) and add it to the compensated total
+// (tot_hi + tot_lo, updated in float64 and split back into two floats).
+template <int DP, int R, int BS>
+__device__ __forceinline__ void store_tile(const double (&acc)[DP][R], int D,
+                                           int B, int T, int64_t j, int b,
+                                           int t0, float *__restrict__ cur,
+                                           float *__restrict__ tot_hi,
+                                           float *__restrict__ tot_lo) {
+  const int64_t dstride = (int64_t)B * T;
 #pragma unroll
   for (int r = 0; r < R; ++r) {
     const int t = t0 + (int)threadIdx.x + BS * r;
@@ -222,6 +225,186 @@ __global__ __launch_bounds__(BS) void exchange_step_kernel(
   }
 }
 
+// Generic plan: every entry carries its own coefficient row coef[e, b, :].
+// The next chunk's gathers are issued before the current chunk's FMAs so
+// that C * R loads per lane are in flight while the VALU works.
+template <int DP, int R, int BS, int C>
+__global__ __launch_bounds__(BS) void exchange_step_kernel(
+    int D, int B, int T, int n_tblk, int64_t nblocks,
+    const int64_t *__restrict__ row_ptr, const int32_t *__restrict__ src,
+    const int32_t *__restrict__ lag, const int32_t *__restrict__ dir,
+    const double *__restrict__ coef, const float *__restrict__ prev,
+    float *__restrict__ cur, float *__restrict__ tot_hi,
+    float *__restrict__ tot_lo) {
+  const int64_t tile = xcd_tile(blockIdx.x, nblocks);
+  const int tb = (int)(tile % n_tblk);
+  const int64_t jb = tile / n_tblk;
+  const int b = (int)(jb % B);
+  const int64_t j = jb / B;
+  const int t0 = tb * (BS * R);
+  const int t_end = min(T, t0 + BS * R);
+
+  double acc[DP][R];
+#pragma unroll
+  for (int d = 0; d < DP; ++d)
+#pragma unroll
+    for (int r = 0; r < R; ++r) acc[d][r] = 0.0;
+
+  const int64_t e_begin = row_ptr[j], e_end = row_ptr[j + 1];
+  if (e_end > e_begin) {
+    auto row_of = [&](int64_t ee) {
+      return (int64_t)src[ee] * D + dir[ee];
+    };
+    float buf[C][R];
+    int n[C];
+    bool act[C];
+    gather_chunk<R, BS, C>(e_begin, e_end, B, T, b, t0, t_end, row_of, lag,
+                           prev, buf, n, act);
+    for (int64_t e = e_begin; e < e_end; e += C) {
+      float nbuf[C][R];
+      int nn[C];
+      bool nact[C];
+      gather_chunk<R, BS, C>(e + C, e_end, B, T, b, t0, t_end, row_of, lag,
+                             prev, nbuf, nn, nact);
+#pragma unroll
+      for (int c = 0; c < C; ++c) {
+        if (!act[c]) continue;
+        const double *cf = coef + ((e + c) * B + b) * DP;
+        double ev[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) ev[r] = (double)buf[c][r];
+#pragma unroll
+        for (int d = 0; d < DP; ++d) {
+          const double cd = cf[d];
+#pragma unroll
+          for (int r = 0; r < R; ++r) acc[d][r] = fma(cd, ev[r], acc[d][r]);
+        }
+      }
+#pragma unroll
+      for (int c = 0; c < C; ++c) {
+        act[c] = nact[c];
+#pragma unroll
+        for (int r = 0; r < R; ++r) buf[c][r] = nbuf[c][r];
+      }
+    }
+  }
+  store_tile<DP, R, BS>(acc, D, B, T, j, b, t0, cur, tot_hi, tot_lo);
+}
+
+// Grouped plan, pass 1 of an order: per group g of destination j's sources
+// (entries sharing one BRDF row), the weighted shifted sum
+//   gsum[g, b, t] = sum_{e in g} scal[b, e] * prev[rowid[e], b, t - lag[e]]
+// in float64, stored as float32.  Only two accumulators per lane: the
+// kernel is a pure gather with many loads in flight.
+template <int R, int BS, int C>
+__global__ __launch_bounds__(BS) void group_gather_kernel(
+    int B, int T, int n_tblk, int64_t nblocks, int64_t Vd,
+    const int64_t *__restrict__ row_ptr, const int64_t *__restrict__ grp_ptr,
+    const int32_t *__restrict__ rowid, const int32_t *__restrict__ lag,
+    const double *__restrict__ scal, const float *__restrict__ prev,
+    float *__restrict__ gsum) {
+  const int64_t tile = xcd_tile(blockIdx.x, nblocks);
+  const int tb = (int)(tile % n_tblk);
+  const int64_t jb = tile / n_tblk;
+  const int b = (int)(jb % B);
+  const int64_t j = jb / B;
+  const int t0 = tb * (BS * R);
+  const int t_end = min(T, t0 + BS * R);
+  const int64_t e_begin = row_ptr[j], e_end = row_ptr[j + 1];
+  if (e_end <= e_begin) return;
+  const double *w = scal + (int64_t)b * Vd;
+  int64_t g = grp_ptr[j];
+  auto row_of = [&](int64_t ee) { return (int64_t)rowid[ee]; };
+  double S[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) S[r] = 0.0;
+  float buf[C][R];
+  int n[C];
+  bool act[C];
+  gather_chunk<R, BS, C>(e_begin, e_end, B, T, b, t0, t_end, row_of, lag,
+                         prev, buf, n, act);
+  for (int64_t e = e_begin; e < e_end; e += C) {
+    float nbuf[C][R];
+    int nn[C];
+    bool nact[C];
+    gather_chunk<R, BS, C>(e + C, e_end, B, T, b, t0, t_end, row_of, lag,
+                           prev, nbuf, nn, nact);
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+      if (e + c >= e_end) continue;
+      if (act[c]) {
+        const double wc = w[e + c];
+#pragma unroll
+        for (int r = 0; r < R; ++r) S[r] = fma(wc, (double)buf[c][r], S[r]);
+      }
+      if (n[c] & kLastInGroup) {
+        float *o = gsum + (g * B + b) * (int64_t)T;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          const int t = t0 + (int)threadIdx.x + BS * r;
+          if (t < T) o[t] = (float)S[r];
+          S[r] = 0.0;
+        }
+        g += 1;
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+      n[c] = nn[c];
+      act[c] = nact[c];
+#pragma unroll
+      for (int r = 0; r < R; ++r) buf[c][r] = nbuf[c][r];
+    }
+  }
+}
+
+// Grouped plan, pass 2: spread the group sums over the D directions with
+// their BRDF rows and update the order histogram and the running total:
+//   cur[j, :, b, t] = sum_{g of j} rtab[key_g, b, :] * gsum[g, b, t].
+template <int DP, int R, int BS>
+__global__ __launch_bounds__(BS) void group_spread_kernel(
+    int D, int B, int T, int n_tblk, int64_t nblocks,
+    const int64_t *__restrict__ grp_ptr, const int32_t *__restrict__ grp_key,
+    const double *__restrict__ rtab, const float *__restrict__ gsum,
+    float *__restrict__ cur, float *__restrict__ tot_hi,
+    float *__restrict__ tot_lo) {
+  const int64_t tile = xcd_tile(blockIdx.x, nblocks);
+  const int tb = (int)(tile % n_tblk);
+  const int64_t jb = tile / n_tblk;
+  const int b = (int)(jb % B);
+  const int64_t j = jb / B;
+  const int t0 = tb * (BS * R);
+  double acc[DP][R];
+#pragma unroll
+  for (int d = 0; d < DP; ++d)
+#pragma unroll
+    for (int r = 0; r < R; ++r) acc[d][r] = 0.0;
+  const int64_t g0 = grp_ptr[j], g1 = grp_ptr[j + 1];
+  float sv[R], nsv[R];
+  auto load = [&](int64_t g, float (&v)[R]) {
+    const float *p = gsum + (g * B + b) * (int64_t)T;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int t = t0 + (int)threadIdx.x + BS * r;
+      v[r] = (g < g1 && t < T) ? p[t] : 0.f;
+    }
+  };
+  if (g1 > g0) load(g0, sv);
+  for (int64_t g = g0; g < g1; ++g) {
+    load(g + 1, nsv);
+    const double *rt = rtab + ((int64_t)grp_key[g] * B + b) * DP;
+#pragma unroll
+    for (int d = 0; d < DP; ++d) {
+      const double cd = rt[d];
+#pragma unroll
+      for (int r = 0; r < R; ++r) acc[d][r] = fma(cd, (double)sv[r], acc[d][r]);
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r) sv[r] = nsv[r];
+  }
+  store_tile<DP, R, BS>(acc, D, B, T, j, b, t0, cur, tot_hi, tot_lo);
+}
+
 template <int DP, int R>
 int launch_step(int64_t P, int64_t D, int64_t B, int64_t T,
                 const int64_t *row_ptr, const int32_t *src,
@@ -232,7 +415,7 @@ int launch_step(int64_t P, int64_t D, int64_t B, int64_t T,
   const int64_t n_tblk = ceil_div(T, (int64_t)BS * R);
   const int64_t nblocks = P * B * n_tblk;
   SPR_REQUIRE(nblocks < (int64_t)0x7fffffff, "exchange grid too large");
-  hipLaunchKernelGGL((exchange_step_kernel<DP, R, BS>),
+  hipLaunchKernelGGL((exchange_step_kernel<DP, R, BS, 4>),
                      dim3((unsigned)nblocks), dim3(BS), 0, s, (int)D, (int)B,
                      (int)T, (int)n_tblk, nblocks, row_ptr, src, lag, dir,
                      coef, prev, cur, tot_hi, tot_lo);
@@ -247,6 +430,181 @@ __global__ void finalize_kernel(int64_t n, const float *__restrict__ hi,
   for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n;
        k += stride)
     out[k] = (float)((double)hi[k] + (double)lo[k]);
+}
+
+// The grouped step: gather-sum then spread.
+constexpr int kGatherR = 2;
+constexpr int kGatherC = 8;
+
+template <int DP, int R>
+int launch_grouped(int64_t P, int64_t D, int64_t B, int64_t T, int64_t Vd,
+                   const int64_t *row_ptr, const int64_t *grp_ptr,
+                   const int32_t *grp_key, const int32_t *rowid,
+                   const int32_t *lag, const double *scal, const double *rtab,
+                   float *gsum, const float *prev, float *cur, float *tot_hi,
+                   float *tot_lo, hipStream_t s) {
+  constexpr int BS = 256;
+  const int64_t nt1 = ceil_div(T, (int64_t)BS * kGatherR);
+  const int64_t nb1 = P * B * nt1;
+  const int64_t nt2 = ceil_div(T, (int64_t)BS * R);
+  const int64_t nb2 = P * B * nt2;
+  SPR_REQUIRE(nb1 < (int64_t)0x7fffffff && nb2 < (int64_t)0x7fffffff,
+              "exchange grid too large");
+  hipLaunchKernelGGL((group_gather_kernel<kGatherR, BS, kGatherC>),
+                     dim3((unsigned)nb1), dim3(BS), 0, s, (int)B, (int)T,
+                     (int)nt1, nb1, Vd, row_ptr, grp_ptr, rowid, lag, scal,
+                     prev, gsum);
+  SPR_LAUNCH_CHECK();
+  hipLaunchKernelGGL((group_spread_kernel<DP, R, BS>), dim3((unsigned)nb2),
+                     dim3(BS), 0, s, (int)D, (int)B, (int)T, (int)nt2, nb2,
+                     grp_ptr, grp_key, rtab, gsum, cur, tot_hi, tot_lo);
+  SPR_LAUNCH_CHECK();
+  return SPR_OK;
+}
+
+// ---------------------------------------------------------------------
+// grouped plan construction
+// ---------------------------------------------------------------------
+constexpr int kSortMax = 8192;  // longest destination row the LDS sort takes
+
+// Key of entry e (source i -> destination j): the BRDF row its coefficient
+// vector comes from, brdf_index[w_i] * S + s_ij (0 without a BRDF).
+__global__ __launch_bounds__(256) void group_key_kernel(
+    int64_t P, const int64_t *__restrict__ row_ptr,
+    const int32_t *__restrict__ src, const double *__restrict__ centers,
+    const int64_t *__restrict__ wall_ids, int64_t S,
+    const int64_t *__restrict__ brdf_index,
+    const double *__restrict__ in_dirs, int32_t *__restrict__ key) {
+  const int64_t j = blockIdx.x;
+  for (int64_t e = row_ptr[j] + threadIdx.x; e < row_ptr[j + 1];
+       e += blockDim.x) {
+    if (!in_dirs) {
+      key[e] = 0;
+      continue;
+    }
+    const int64_t i = src[e];
+    d3 u = load3(centers + 3 * i) - load3(centers + 3 * j);
+    u = div(u, norm(u));
+    const int64_t w = wall_ids[i];
+    const int64_t sidx = nearest_direction(in_dirs + w * S * 3, S, u);
+    key[e] = (int32_t)(brdf_index[w] * S + sidx);
+  }
+}
+
+// Sort each destination row by (key, position) with a bitonic sort in LDS
+// (stable: ties keep ascending source order) and count its groups.
+__global__ __launch_bounds__(1024) void group_sort_kernel(
+    int64_t P, const int64_t *__restrict__ row_ptr, int32_t *__restrict__ src,
+    int32_t *__restrict__ key, int64_t *__restrict__ grp_count, int npad) {
+  // dynamic LDS only (8-byte aligned base): keys, source ids, group count
+  extern __shared__ uint64_t smem64[];
+  uint64_t *k = smem64;
+  int32_t *sv = reinterpret_cast<int32_t *>(k + npad);
+  int &groups = sv[npad];
+  const int64_t j = blockIdx.x;
+  const int64_t e0 = row_ptr[j];
+  const int n = (int)(row_ptr[j + 1] - e0);
+  for (int q = threadIdx.x; q < npad; q += blockDim.x) {
+    if (q < n) {
+      k[q] = ((uint64_t)(uint32_t)key[e0 + q] << 32) | (uint32_t)q;
+      sv[q] = src[e0 + q];
+    } else {
+      k[q] = ~0ull;
+    }
+  }
+  if (threadIdx.x == 0) groups = 0;
+  __syncthreads();
+  for (int size = 2; size <= npad; size <<= 1) {
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      for (int q = threadIdx.x; q < npad; q += blockDim.x) {
+        int p = q ^ stride;
+        if (p > q) {
+          bool up = (q & size) == 0;
+          uint64_t a = k[q], c = k[p];
+          if ((a > c) == up) {
+            k[q] = c;
+            k[p] = a;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+  int local = 0;
+  for (int q = threadIdx.x; q < n; q += blockDim.x) {
+    const uint32_t kq = (uint32_t)(k[q] >> 32);
+    src[e0 + q] = sv[(uint32_t)(k[q] & 0xffffffffu)];
+    key[e0 + q] = (int32_t)kq;
+    if (q == 0 || (uint32_t)(k[q - 1] >> 32) != kq) local += 1;
+  }
+  atomicAdd(&groups, local);
+  __syncthreads();
+  if (threadIdx.x == 0) grp_count[j] = groups;
+}
+
+// Group starts and keys, one wavefront per destination row.
+__global__ __launch_bounds__(256) void group_fill_kernel(
+    int64_t P, int64_t Vd, const int64_t *__restrict__ row_ptr,
+    const int32_t *__restrict__ key, const int64_t *__restrict__ grp_ptr,
+    int64_t *__restrict__ grp_start, int32_t *__restrict__ grp_key) {
+  const int lane = threadIdx.x & 63;
+  const int64_t j = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (j >= P) return;
+  const int64_t e0 = row_ptr[j], e1 = row_ptr[j + 1];
+  int64_t pos = grp_ptr[j];
+  for (int64_t base = e0; base < e1; base += 64) {
+    const int64_t e = base + lane;
+    const bool f = e < e1 && (e == e0 || key[e] != key[e - 1]);
+    const uint64_t m = __ballot(f);
+    if (f) {
+      const uint64_t lt = lane ? (m & ((1ull << lane) - 1ull)) : 0ull;
+      const int64_t g = pos + __popcll(lt);
+      grp_start[g] = e;
+      grp_key[g] = key[e];
+    }
+    pos += __popcll(m);
+  }
+  if (j == P - 1 && lane == 0) grp_start[grp_ptr[P]] = Vd;
+}
+
+// Per-entry delay (with the last-in-group flag), source row and per-band
+// scalar F~_ij * exp(-air_b |u|), stored band-major: scal[b * Vd + e].
+__global__ __launch_bounds__(256) void group_entry_kernel(
+    int64_t P, int64_t D, int64_t B, double c, double dt, int64_t Vd,
+    const int64_t *__restrict__ row_ptr, const int32_t *__restrict__ src,
+    const int32_t *__restrict__ key, const double *__restrict__ centers,
+    const double *__restrict__ areas, const double *__restrict__ ff,
+    const int64_t *__restrict__ p2o, const double *__restrict__ air,
+    int32_t *__restrict__ rowid, int32_t *__restrict__ lag,
+    double *__restrict__ scal) {
+  const int64_t j = blockIdx.x;
+  const int64_t e1 = row_ptr[j + 1];
+  for (int64_t e = row_ptr[j] + threadIdx.x; e < e1; e += blockDim.x) {
+    const int64_t i = src[e];
+    const double dij = norm(load3(centers + 3 * i) - load3(centers + 3 * j));
+    const bool last = e + 1 == e1 || key[e + 1] != key[e];
+    lag[e] = clamp_lag(delay_floor(dij, c, dt)) | (last ? kLastInGroup : 0);
+    rowid[e] = (int32_t)(i * D + checked_dir(p2o[i * P + j], D));
+    PairFactor pf = pair_factor(i, j, P, centers, areas, ff, nullptr, 1, 1, B,
+                                nullptr, nullptr, nullptr);
+    for (int64_t b = 0; b < B; ++b) {
+      double v = pf.f;
+      if (air) v = v * exp(-air[b] * pf.distance);
+      scal[b * Vd + e] = v;
+    }
+  }
+}
+
+// rtab[(bi*S + s), b, d] = brdf[bi, s, d, b] (zero-padded to DP).
+__global__ void rtab_kernel(int64_t NK, int64_t D, int64_t B, int64_t DP,
+                            const double *__restrict__ brdf,
+                            double *__restrict__ rtab) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= NK * B * DP) return;
+  const int64_t d = q % DP, b = (q / DP) % B, k = q / (DP * B);
+  double v = 0.0;
+  if (d < D) v = brdf ? brdf[(k * D + d) * B + b] : 1.0;
+  rtab[q] = v;
 }
 
 inline size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
@@ -397,6 +755,105 @@ extern "C" int spr_exchange_finalize(int64_t n, const float *tot_hi,
                      as_stream(stream), n, tot_hi, tot_lo, etc);
   SPR_LAUNCH_CHECK();
   return SPR_OK;
+}
+
+extern "C" int spr_plan_grouped_sort(
+    int64_t P, int64_t max_row, const int64_t *row_ptr, int32_t *src,
+    const double *centers, const int64_t *wall_ids, int64_t S,
+    const int64_t *brdf_index, const double *in_dirs, int32_t *key,
+    int64_t *grp_ptr, void *stream) {
+  SPR_REQUIRE(P > 0 && max_row >= 0, "spr_plan_grouped_sort: bad sizes");
+  SPR_REQUIRE(row_ptr && src && centers && key && grp_ptr,
+              "spr_plan_grouped_sort: null pointer");
+  SPR_REQUIRE(!in_dirs || (wall_ids && brdf_index && S > 0),
+              "spr_plan_grouped_sort: in_dirs needs wall_ids, brdf_index");
+  if (max_row > kSortMax)
+    return fail(SPR_ERR_UNSUPPORTED,
+                "spr_plan_grouped_sort: a destination sees %lld patches, "
+                "the grouped plan supports %d",
+                (long long)max_row, kSortMax);
+  hipStream_t s = as_stream(stream);
+  hipLaunchKernelGGL(group_key_kernel, dim3((unsigned)P), dim3(256), 0, s, P,
+                     row_ptr, src, centers, wall_ids, S, brdf_index, in_dirs,
+                     key);
+  SPR_LAUNCH_CHECK();
+  int npad = 1;
+  while (npad < max_row) npad <<= 1;
+  const size_t lds =
+      (size_t)npad * (sizeof(uint64_t) + sizeof(int32_t)) + sizeof(int32_t);
+  SPR_HIP(hipFuncSetAttribute((const void *)group_sort_kernel,
+                              hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)lds));
+  hipLaunchKernelGGL(group_sort_kernel, dim3((unsigned)P), dim3(1024), lds, s,
+                     P, row_ptr, src, key, grp_ptr, npad);
+  SPR_LAUNCH_CHECK();
+  return scan_exclusive(grp_ptr, P, s);
+}
+
+extern "C" int spr_plan_grouped_fill(
+    int64_t P, int64_t D, int64_t B, double c, double dt, int64_t Vd,
+    const int64_t *row_ptr, const int32_t *src, const int32_t *key,
+    const int64_t *grp_ptr, const double *centers, const double *areas,
+    const double *ff, const int64_t *p2o, const double *air, int64_t NK,
+    const double *brdf, int64_t *grp_start, int32_t *grp_key, int32_t *rowid,
+    int32_t *lag, double *scal, double *rtab, void *stream) {
+  const int64_t DP = spr_padded_directions(D);
+  SPR_REQUIRE(DP > 0, "spr_plan_grouped_fill: D=%lld unsupported (max 64)",
+              (long long)D);
+  SPR_REQUIRE(P > 0 && B > 0 && NK > 0 && c > 0 && dt > 0 && Vd >= 0,
+              "spr_plan_grouped_fill: bad sizes");
+  SPR_REQUIRE(row_ptr && src && key && grp_ptr && centers && areas && ff &&
+                  p2o && grp_start && grp_key && rowid && lag && scal && rtab,
+              "spr_plan_grouped_fill: null pointer");
+  SPR_REQUIRE(brdf || (D == 1 && NK == 1),
+              "spr_plan_grouped_fill: without a BRDF D and NK must be 1");
+  hipStream_t s = as_stream(stream);
+  hipLaunchKernelGGL(group_fill_kernel, dim3((unsigned)ceil_div(P, 4)),
+                     dim3(256), 0, s, P, Vd, row_ptr, key, grp_ptr, grp_start,
+                     grp_key);
+  SPR_LAUNCH_CHECK();
+  hipLaunchKernelGGL(group_entry_kernel, dim3((unsigned)P), dim3(256), 0, s,
+                     P, D, B, c, dt, Vd, row_ptr, src, key, centers, areas, ff,
+                     p2o, air, rowid, lag, scal);
+  SPR_LAUNCH_CHECK();
+  const int64_t nr = NK * B * DP;
+  hipLaunchKernelGGL(rtab_kernel, dim3((unsigned)ceil_div(nr, 256)),
+                     dim3(256), 0, s, NK, D, B, DP, brdf, rtab);
+  SPR_LAUNCH_CHECK();
+  return SPR_OK;
+}
+
+extern "C" int spr_exchange_step_grouped(
+    int64_t P, int64_t D, int64_t B, int64_t T, int64_t Vd,
+    const int64_t *row_ptr, const int64_t *grp_ptr, const int32_t *grp_key,
+    const int32_t *rowid, const int32_t *lag, const double *scal,
+    const double *rtab, float *gsum, const float *prev, float *cur,
+    float *tot_hi, float *tot_lo, void *stream) {
+  const int64_t DP = spr_padded_directions(D);
+  SPR_REQUIRE(DP > 0, "spr_exchange_step_grouped: D=%lld unsupported",
+              (long long)D);
+  SPR_REQUIRE(P > 0 && B > 0 && T > 0 && T < 0x3fffffff && Vd >= 0,
+              "spr_exchange_step_grouped: bad sizes");
+  SPR_REQUIRE(row_ptr && grp_ptr && grp_key && rowid && lag && scal && rtab &&
+                  gsum && prev && cur && tot_hi && tot_lo,
+              "spr_exchange_step_grouped: null pointer");
+  SPR_REQUIRE(prev != cur, "spr_exchange_step_grouped: prev == cur");
+  hipStream_t s = as_stream(stream);
+#define SPR_GSTEP(DPV, RV)                                                    \
+  return launch_grouped<DPV, RV>(P, D, B, T, Vd, row_ptr, grp_ptr, grp_key,   \
+                                 rowid, lag, scal, rtab, gsum, prev, cur,     \
+                                 tot_hi, tot_lo, s)
+  switch (DP) {
+    case 1: SPR_GSTEP(1, 8);
+    case 2: SPR_GSTEP(2, 8);
+    case 4: SPR_GSTEP(4, 8);
+    case 8: SPR_GSTEP(8, 8);
+    case 16: SPR_GSTEP(16, 4);
+    case 32: SPR_GSTEP(32, 2);
+    case 64: SPR_GSTEP(64, 1);
+  }
+#undef SPR_GSTEP
+  return fail(SPR_ERR_UNSUPPORTED, "spr_exchange_step_grouped: bad D");
 }
 
 namespace {

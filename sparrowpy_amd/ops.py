@@ -22,7 +22,7 @@ __all__ = [
     "patch2patch_ff_universal", "direction_map", "point_direction_index",
     "form_factors_with_directivity_dim", "source_energy",
     "receiver_energy", "collect", "energy_exchange", "ExchangePlan",
-    "ExchangeState",
+    "GroupedPlan", "ExchangeState",
 ]
 
 
@@ -353,6 +353,90 @@ class ExchangePlan:
         return plan
 
 
+@dataclasses.dataclass
+class GroupedPlan:
+    """Destination-major pair table with sources grouped by BRDF row.
+
+    ``rtab[k, b, :]`` is the BRDF row k = bi*S + s; ``scal[b, e]`` the
+    per-pair weight F~_ij * exp(-air_b |u_ij|); ``lag`` carries the
+    last-in-group flag in bit 30; see sparrow_hip.h.
+    """
+
+    LAST_IN_GROUP = 0x40000000
+
+    P: int
+    D: int
+    B: int
+    row_ptr: torch.Tensor
+    grp_ptr: torch.Tensor
+    grp_start: torch.Tensor
+    grp_key: torch.Tensor
+    rowid: torch.Tensor
+    lag: torch.Tensor
+    scal: torch.Tensor
+    rtab: torch.Tensor
+
+    @property
+    def Vd(self):
+        return int(self.rowid.numel())
+
+    @property
+    def n_groups(self):
+        return int(self.grp_key.numel())
+
+    MAX_ROW = 8192
+
+    @classmethod
+    def from_geometry(cls, vis, patches_center, patches_area, form_factors,
+                      p2o, air_attenuation, patch_to_wall_ids, sources,
+                      scattering, scattering_index, n_directions, n_bins,
+                      speed_of_sound, time_resolution):
+        c = _f64(patches_center)
+        dev = c.device
+        P = c.shape[0]
+        D, B = int(n_directions), int(n_bins)
+        DP = int(_hip.lib().spr_padded_directions(D))
+        if DP == 0:
+            raise ValueError(f"{D} directions unsupported (max 64)")
+        row_ptr, src = ExchangePlan.rows(vis)
+        Vd = src.numel()
+        max_row = int((row_ptr[1:] - row_ptr[:-1]).max().item()) if P else 0
+        if max_row > cls.MAX_ROW:
+            raise ValueError("destination rows too long for the grouped plan")
+        src = src.clone() if Vd else torch.empty(1, dtype=torch.int32,
+                                                 device=dev)
+        w = _i64(patch_to_wall_ids)
+        if scattering is None:
+            S, NK, brdf, bidx, sd = 1, 1, None, None, None
+        else:
+            sd = _f64(sources)
+            S = sd.shape[1]
+            brdf = _f64(scattering)
+            bidx = _i64(scattering_index)
+            NK = brdf.shape[0] * S
+        key = torch.empty(max(Vd, 1), dtype=torch.int32, device=dev)
+        grp_ptr = torch.empty(P + 1, dtype=torch.int64, device=dev)
+        call("spr_plan_grouped_sort", P, max_row, ptr(row_ptr), ptr(src),
+             ptr(c), ptr(w), S, ptr(bidx), ptr(sd), ptr(key), ptr(grp_ptr),
+             stream_ptr())
+        G = int(grp_ptr[P].item())
+        grp_start = torch.empty(G + 1, dtype=torch.int64, device=dev)
+        grp_key = torch.empty(max(G, 1), dtype=torch.int32, device=dev)[:G]
+        rowid = torch.empty(max(Vd, 1), dtype=torch.int32, device=dev)
+        lag = torch.empty(max(Vd, 1), dtype=torch.int32, device=dev)
+        scal = torch.empty((B, max(Vd, 1)), dtype=torch.float64, device=dev)
+        rtab = torch.empty((NK, B, DP), dtype=torch.float64, device=dev)
+        call("spr_plan_grouped_fill", P, D, B, float(speed_of_sound),
+             float(time_resolution), Vd, ptr(row_ptr), ptr(src), ptr(key),
+             ptr(grp_ptr), ptr(c), ptr(_f64(patches_area)),
+             ptr(_f64(form_factors)), ptr(_i64(p2o)),
+             ptr(_f64(air_attenuation)), NK, ptr(brdf), ptr(grp_start),
+             ptr(grp_key), ptr(rowid), ptr(lag), ptr(scal), ptr(rtab),
+             stream_ptr())
+        return cls(P, D, B, row_ptr, grp_ptr, grp_start, grp_key,
+                   rowid[:Vd], lag[:Vd], scal[:, :Vd].contiguous(), rtab)
+
+
 class ExchangeState:
     """Order histograms and the compensated running total of one source.
 
@@ -368,6 +452,7 @@ class ExchangeState:
         self.hi = torch.empty(shape, dtype=torch.float32, device=dev)
         self.lo = torch.empty(shape, dtype=torch.float32, device=dev)
         self.err = torch.zeros(1, dtype=torch.int32, device=dev)
+        self.gsum = None  # per-group sums of the grouped plan (workspace)
 
     def init(self, energy_0_directivity, distance_0, speed_of_sound,
              time_resolution):
@@ -380,13 +465,26 @@ class ExchangeState:
              float(speed_of_sound), float(time_resolution), ptr(self.prev),
              ptr(self.hi), ptr(self.lo), ptr(self.err), stream_ptr())
 
-    def step(self, plan: ExchangePlan):
-        """One reflection order; swaps prev and cur."""
+    def step(self, plan):
+        """One reflection order on an ExchangePlan or GroupedPlan; swaps
+        prev and cur."""
         P, D, B, T = self.shape
-        call("spr_exchange_step", P, D, B, T, ptr(plan.row_ptr),
-             ptr(plan.src), ptr(plan.lag), ptr(plan.dir), ptr(plan.coef),
-             ptr(self.prev), ptr(self.cur), ptr(self.hi), ptr(self.lo),
-             stream_ptr())
+        if isinstance(plan, GroupedPlan):
+            G = plan.n_groups
+            if self.gsum is None or self.gsum.shape[0] < max(G, 1):
+                self.gsum = torch.empty((max(G, 1), B, T),
+                                        dtype=torch.float32,
+                                        device=self.prev.device)
+            call("spr_exchange_step_grouped", P, D, B, T, plan.Vd,
+                 ptr(plan.row_ptr), ptr(plan.grp_ptr), ptr(plan.grp_key),
+                 ptr(plan.rowid), ptr(plan.lag), ptr(plan.scal),
+                 ptr(plan.rtab), ptr(self.gsum), ptr(self.prev),
+                 ptr(self.cur), ptr(self.hi), ptr(self.lo), stream_ptr())
+        else:
+            call("spr_exchange_step", P, D, B, T, ptr(plan.row_ptr),
+                 ptr(plan.src), ptr(plan.lag), ptr(plan.dir), ptr(plan.coef),
+                 ptr(self.prev), ptr(self.cur), ptr(self.hi), ptr(self.lo),
+                 stream_ptr())
         self.prev, self.cur = self.cur, self.prev
 
     def finalize(self, out=None):

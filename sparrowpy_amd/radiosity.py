@@ -254,16 +254,37 @@ class DirectionalRadiosityFast:
         self._dev["distance_patches_to_source"] = d0
 
     # -- exchange -----------------------------------------------------------
-    def _exchange_plan(self, speed_of_sound, time_resolution):
-        key = (float(speed_of_sound), float(time_resolution))
+    # Use the grouped plan when groups are at most this fraction of pairs
+    # (each group costs D FMAs per bin, each pair one FMA).
+    GROUPED_MAX_GROUP_FRACTION = 0.25
+
+    def _exchange_plan(self, speed_of_sound, time_resolution, kind=None):
+        """Plan of the exchange (cached per speed of sound and bin width).
+
+        ``kind``: "grouped", "generic" or None (automatic: grouped when the
+        sources of a destination share few BRDF rows).
+        """
+        key = (float(speed_of_sound), float(time_resolution), kind)
         if getattr(self, "_plan", None) is not None and self._plan[0] == key:
             return self._plan[1]
         b = self._bake
         g = self._geometry()
-        plan = ops.ExchangePlan.from_geometry(
-            b["vis"], g["centers"], g["areas"], b["ff"], b["p2o"], b["air"],
-            g["wall_ids"], b["sources"], b["brdf"], b["brdf_index"], b["D"],
-            b["n_bins"], speed_of_sound, time_resolution)
+        args = (b["vis"], g["centers"], g["areas"], b["ff"], b["p2o"],
+                b["air"], g["wall_ids"], b["sources"], b["brdf"],
+                b["brdf_index"], b["D"], b["n_bins"], speed_of_sound,
+                time_resolution)
+        plan = None
+        if kind in (None, "grouped"):
+            try:
+                plan = ops.GroupedPlan.from_geometry(*args)
+            except ValueError:
+                if kind == "grouped":
+                    raise
+            if (plan is not None and kind is None and plan.Vd and
+                    plan.n_groups > self.GROUPED_MAX_GROUP_FRACTION * plan.Vd):
+                plan = None
+        if plan is None:
+            plan = ops.ExchangePlan.from_geometry(*args)
         self._plan = (key, plan)
         return plan
 

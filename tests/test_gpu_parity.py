@@ -125,6 +125,7 @@ def _oracle_directional(walls, ps, **kw):
     return r
 
 
+
 @pytest.fixture(scope="module")
 def dshoebox():
     return _oracle_directional(testing.shoebox_room_stub(3, 2.5, 2), .5)
@@ -247,3 +248,104 @@ def test_collect(ops, dshoebox):
     mono = _np(ops.collect(etc32, recv, r.patches_center, idx, g, r.air,
                            343., 1 / 2000, patchwise=False))
     assert rel_err_within_db(mono, ref_pw.sum(axis=1)) <= EXCHANGE_RTOL
+
+
+def _grouped_and_generic(ops, r, c, dt):
+    args = (r.visibility_matrix, r.patches_center, r.patches_area,
+            r.form_factors, r.p2o, r.air, r.patch_to_wall_ids, r.brdf_in,
+            np.array(r.brdf), r.brdf_index, r.brdf_out.shape[1], r.n_bins, c,
+            dt)
+    return ops.GroupedPlan.from_geometry(*args), \
+        ops.ExchangePlan.from_geometry(*args)
+
+
+@pytest.mark.parametrize("per_wall", [True, False])
+def test_grouped_plan_contents(ops, per_wall):
+    """Every row of the grouped plan holds the generic row's entries: same
+    (source row, delay) multiset, and scal * rtab[key] equals the generic
+    coefficient row."""
+    r = _oracle_directional(testing.shoebox_room_stub(3, 2.5, 2), .5,
+                            per_wall=per_wall, seed=3)
+    c, dt = 343., 1 / 2000
+    gp, gen = _grouped_and_generic(ops, r, c, dt)
+    P, D, B = r.n_patches, r.brdf_out.shape[1], r.n_bins
+    np.testing.assert_array_equal(_np(gp.row_ptr), _np(gen.row_ptr))
+    gs, gk = _np(gp.grp_start), _np(gp.grp_key)
+    gptr = _np(gp.grp_ptr)
+    assert gs[0] == 0 and gs[-1] == gp.Vd and np.all(np.diff(gs) > 0)
+    rp = _np(gen.row_ptr)
+    assert np.all(np.isin(rp[:-1], gs))
+    np.testing.assert_array_equal(gs[gptr[:-1]], rp[:-1])
+    g_of_e = np.searchsorted(gs, np.arange(gp.Vd), side="right") - 1
+    DP = gp.rtab.shape[-1]
+    coef_g = _np(gp.scal).T[:, :, None] * _np(gp.rtab)[gk[g_of_e]]
+    a = _np(gen.src).astype(np.int64) * D + _np(gen.dir)
+    b = _np(gp.rowid).astype(np.int64)
+    cg, lg = _np(gen.coef), _np(gen.lag)
+    flag = ops.GroupedPlan.LAST_IN_GROUP
+    lp = _np(gp.lag) & (flag - 1)
+    # the flag marks exactly the last entry of every group
+    last = np.nonzero(_np(gp.lag) & flag)[0]
+    np.testing.assert_array_equal(last, gs[1:] - 1)
+    for j in range(P):
+        s = slice(rp[j], rp[j + 1])
+        oa, ob = np.argsort(a[s], kind="stable"), np.argsort(b[s],
+                                                            kind="stable")
+        np.testing.assert_array_equal(a[s][oa], b[s][ob])
+        np.testing.assert_array_equal(lg[s][oa], lp[s][ob])
+        np.testing.assert_allclose(coef_g[s][ob], cg[s][oa], rtol=1e-14,
+                                   atol=0)
+        assert gptr[j + 1] - gptr[j] <= rp[j + 1] - rp[j]
+
+
+@pytest.mark.parametrize("per_wall", [True, False])
+def test_grouped_plan_matches_oracle(ops, per_wall):
+    """Grouped plan (sources sharing a BRDF row summed before the D-wide
+    spread) against the oracle's generic FP64 exchange."""
+    r = _oracle_directional(testing.shoebox_room_stub(3, 2.5, 2), .5,
+                            per_wall=per_wall, seed=3)
+    src = np.array([1.1, .8, .6])
+    r.init_source_energy(src)
+    T, c, dt, K = 400, 343., 1 / 2000, 8
+    plan, gen = _grouped_and_generic(ops, r, c, dt)
+    assert plan.n_groups < plan.Vd
+    P, D, B = r.energy_init_source.shape
+    outs = []
+    for pl in (gen, plan):
+        st = ops.ExchangeState(P, D, B, T)
+        st.init(r.energy_init_source, r.distance_patches_to_source, c, dt)
+        for _ in range(K):
+            st.step(pl)
+        outs.append(_np(st.finalize()))
+        st.check()
+    ref = orc.exchange(T, r.energy_init_source, r.distance_patches_to_source,
+                       r.pair_distances(), r.fft, r.p2o, c, dt, K,
+                       r.visible_patches, nthreads=8)
+    e_gen = rel_err_within_db(outs[0], ref)
+    e_grp = rel_err_within_db(outs[1], ref)
+    print(f"[grouped per_wall={per_wall}] groups {plan.n_groups} of "
+          f"{plan.Vd} entries, rel err generic {e_gen:.3e} "
+          f"grouped {e_grp:.3e}")
+    assert e_gen <= EXCHANGE_RTOL
+    assert e_grp <= EXCHANGE_RTOL
+
+
+def test_grouped_plan_without_brdf(ops):
+    """No BRDF set before the bake: D = 1, one group per destination."""
+    r = make_oracle(testing.shoebox_room_stub(2, 2, 1), .5)
+    r.bake_geometry()
+    r.init_source_energy([.7, .6, .4])
+    T, c, dt, K = 200, 343., 1 / 2000, 5
+    plan = ops.GroupedPlan.from_geometry(
+        r.visibility_matrix, r.patches_center, r.patches_area,
+        r.form_factors, r.p2o, None, r.patch_to_wall_ids, None, None, None,
+        1, 1, c, dt)
+    assert plan.n_groups == r.n_patches
+    st = ops.ExchangeState(r.n_patches, 1, 1, T)
+    st.init(r.energy_init_source, r.distance_patches_to_source, c, dt)
+    for _ in range(K):
+        st.step(plan)
+    ref = orc.exchange(T, r.energy_init_source, r.distance_patches_to_source,
+                       r.pair_distances(), r.fft, r.p2o, c, dt, K,
+                       r.visible_patches)
+    assert rel_err_within_db(_np(st.finalize()), ref) <= EXCHANGE_RTOL
