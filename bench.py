@@ -59,13 +59,14 @@ def _generic_rows(plan, D):
     row_ptr = plan.row_ptr.cpu().numpy()
     lag = plan.lag.cpu().numpy()
     if isinstance(plan, ops.GroupedPlan):
+        lag = lag & (ops.GroupedPlan.LAST_IN_GROUP - 1)
         rowid = plan.rowid.cpu().numpy().astype(np.int64)
         src, dirs = rowid // D, rowid % D
         gs = plan.grp_start.cpu().numpy()
         gk = plan.grp_key.cpu().numpy()
         g_of_e = np.searchsorted(gs, np.arange(rowid.size), side="right") - 1
         rt = plan.rtab[:, 0, :D].cpu().numpy()
-        coef = plan.scal[:, 0].cpu().numpy()[:, None] * rt[gk[g_of_e]]
+        coef = plan.scal[0].cpu().numpy()[:, None] * rt[gk[g_of_e]]
     else:
         src = plan.src.cpu().numpy()
         dirs = plan.dir.cpu().numpy()
@@ -117,6 +118,20 @@ def cpu_baseline(plan, D, T, seconds, seed=0):
     return upd / dt, nthreads, T_s, n_dest
 
 
+def copy_calibration(n_bytes=1 << 32, reps=3):
+    """Copy n_bytes device-to-device ``reps`` times (one kernel each)."""
+    import torch
+    src = torch.ones(n_bytes // 4, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    for _ in range(reps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": n_bytes, "copy_bytes_written":
+                      n_bytes, "reps": reps}), flush=True)
+    return 0
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -131,7 +146,12 @@ def main(argv=None):
                     choices=["auto", "grouped", "generic"],
                     help="exchange plan (auto: grouped when sources share "
                          "BRDF rows)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run only a device copy of known size (to scale "
+                         "the FETCH_SIZE/WRITE_SIZE counters)")
     args = ap.parse_args(argv)
+    if args.copy_calibration:
+        return copy_calibration()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not "
               "supported (the exchange shards by band with no collectives; "
@@ -186,23 +206,32 @@ def main(argv=None):
     torch.cuda.synchronize()
     log(f"warmup {args.warmup} orders done")
 
-    lag = plan.lag.to(torch.int64)
+    lag = plan.lag.to(torch.int64) & (ops.GroupedPlan.LAST_IN_GROUP - 1)
     U = int((B * torch.clamp(T - lag, min=0)).sum().item())
     Vd = plan.Vd
     stream = torch.cuda.current_stream()
-    starts = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
-    ends = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+    n_ev = 3 if kind == "grouped" else 2
+    ev = [[torch.cuda.Event(enable_timing=True) for _ in range(n_ev)]
+          for _ in range(args.steps)]
     torch.cuda.synchronize()
     wall0 = time.perf_counter()
     for k in range(args.steps):
-        starts[k].record(stream)
-        state.step(plan)
-        ends[k].record(stream)
+        ev[k][0].record(stream)
+        if kind == "grouped":
+            state.gather(plan)
+            ev[k][1].record(stream)
+            state.spread(plan)
+            state.prev, state.cur = state.cur, state.prev
+        else:
+            state.step(plan)
+        ev[k][-1].record(stream)
     torch.cuda.synchronize()
     wall = time.perf_counter() - wall0
-    kern_ms = [s.elapsed_time(e) for s, e in zip(starts, ends)]
     log(f"timed {args.steps} orders: {wall:.3f} s")
-    avg_s = float(np.mean(kern_ms)) / 1e3
+    step_s = float(np.mean([e[0].elapsed_time(e[-1]) for e in ev])) / 1e3
+    if kind == "grouped":
+        gather_s = float(np.mean([e[0].elapsed_time(e[1]) for e in ev])) / 1e3
+        spread_s = float(np.mean([e[1].elapsed_time(e[2]) for e in ev])) / 1e3
     state.check()
 
     # collection after the timed region (reported, not part of value)
@@ -217,11 +246,30 @@ def main(argv=None):
 
     value = U * args.steps / wall
     n_state = P * D * B * T
-    # algorithmic bytes per order: gathered source bins, the F~ slice and
-    # delay/direction of each directed pair, one write of the order's
-    # histogram and a read + write of the running total
-    alg_bytes = 4 * U + Vd * (4 * D * B + 8) + 12 * n_state
-    achieved_gbs = alg_bytes / avg_s / 1e9
+    # algorithmic bytes per order (reference accounting): gathered source
+    # bins, the F~ slice and delay/direction of each directed pair, one write
+    # of the order's histogram and a read + write of the running total
+    order_bytes = 4 * U + Vd * (4 * D * B + 8) + 12 * n_state
+    kernels = {}
+    if kind == "grouped":
+        G = plan.n_groups
+        # gather: gathered bins, per-pair row/delay + per-band weight, one
+        # write of the per-group sums
+        gather_bytes = 4 * U + Vd * (8 + 8 * B) + 4 * G * B * T
+        # spread: read the group sums, write cur, read+write hi and lo
+        spread_bytes = 4 * G * B * T + 20 * n_state
+        kernels = {
+            "group_gather_kernel": {
+                "ms": gather_s * 1e3, "algorithmic_bytes": gather_bytes,
+                "GB/s": gather_bytes / gather_s / 1e9},
+            "group_spread_kernel": {
+                "ms": spread_s * 1e3, "algorithmic_bytes": spread_bytes,
+                "GB/s": spread_bytes / spread_s / 1e9},
+        }
+        dom, dom_s, dom_bytes = "group_gather_kernel", gather_s, gather_bytes
+    else:
+        dom, dom_s, dom_bytes = "exchange_step_kernel", step_s, order_bytes
+    achieved_gbs = dom_bytes / dom_s / 1e9
     flops = 2 * D * U
     line = {
         "metric": "patch-pair·bin updates/s at 5k patches",
@@ -248,11 +296,15 @@ def main(argv=None):
             "bound": "hbm", "achieved": achieved_gbs, "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": achieved_gbs / HBM_PEAK_GBS,
             "traffic": None,
-            "kernel": "exchange_grouped_kernel" if kind == "grouped"
-                      else "exchange_step_kernel",
-            "kernel_ms": avg_s * 1e3,
-            "algorithmic_bytes": alg_bytes,
-            "valu_tflops": flops / avg_s / 1e12,
+            "kernel": dom, "kernel_ms": dom_s * 1e3,
+            "algorithmic_bytes": dom_bytes,
+        },
+        "order": {
+            "ms": step_s * 1e3, "algorithmic_bytes": order_bytes,
+            "GB/s": order_bytes / step_s / 1e9,
+            "frac_hbm": order_bytes / step_s / 1e9 / HBM_PEAK_GBS,
+            "reference_flops_tflops": flops / step_s / 1e12,
+            "kernels": kernels,
         },
         "setup": t,
     }

@@ -246,6 +246,22 @@ int spr_plan_grouped_fill(int64_t P, int64_t D, int64_t B, double c,
                           int32_t *grp_key, int32_t *rowid, int32_t *lag,
                           double *scal, double *rtab, void *stream);
 
+/* The two passes of spr_exchange_step_grouped, separately:
+ * gsum[g,b,t] = sum_{e in g} scal[b,e] * prev[rowid[e], b, t - lag[e]]. */
+int spr_exchange_gather_grouped(int64_t P, int64_t B, int64_t T, int64_t Vd,
+                                const int64_t *row_ptr,
+                                const int64_t *grp_ptr, const int32_t *rowid,
+                                const int32_t *lag, const double *scal,
+                                const float *prev, float *gsum, void *stream);
+
+/* cur[j,:,b,t] = sum_{g of j} rtab[grp_key[g], b, :] * gsum[g,b,t] and the
+ * compensated running total tot_hi + tot_lo += cur. */
+int spr_exchange_spread_grouped(int64_t P, int64_t D, int64_t B, int64_t T,
+                                const int64_t *grp_ptr,
+                                const int32_t *grp_key, const double *rtab,
+                                const float *gsum, float *cur, float *tot_hi,
+                                float *tot_lo, void *stream);
+
 /* One reflection order on a grouped plan (same result as
  * spr_exchange_step on the equivalent generic plan): a gather pass writes
  * the per-group sums gsum[G,B,T] (float32 workspace), a spread pass turns

@@ -81,6 +81,10 @@ SYMBOLS = {
                                      _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
                                      _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
                                      _vp]),
+    "spr_exchange_gather_grouped": (_int, [_i64, _i64, _i64, _i64, _vp, _vp,
+                                           _vp, _vp, _vp, _vp, _vp, _vp]),
+    "spr_exchange_spread_grouped": (_int, [_i64, _i64, _i64, _i64, _vp, _vp,
+                                           _vp, _vp, _vp, _vp, _vp, _vp]),
     "spr_exchange_step_grouped": (_int, [_i64, _i64, _i64, _i64, _i64, _vp,
                                          _vp, _vp, _vp, _vp, _vp, _vp, _vp,
                                          _vp, _vp, _vp, _vp, _vp]),

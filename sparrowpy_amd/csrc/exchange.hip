@@ -291,6 +291,20 @@ __global__ __launch_bounds__(BS) void exchange_step_kernel(
   store_tile<DP, R, BS>(acc, D, B, T, j, b, t0, cur, tot_hi, tot_lo);
 }
 
+// Tile order of the grouped passes: destination fastest, then time block,
+// then band.  With the XCD-contiguous block mapping, the blocks resident on
+// one XCD at a time are neighbouring destinations (neighbouring patches of
+// a wall) in the same band and time window, which read nearly the same
+// source histogram segments -- so those reads are shared through L2/MALL.
+__device__ __forceinline__ void dest_major_tile(int64_t tile, int64_t P,
+                                                int n_tblk, int64_t &j,
+                                                int &b, int &tb) {
+  j = tile % P;
+  const int64_t rest = tile / P;
+  tb = (int)(rest % n_tblk);
+  b = (int)(rest / n_tblk);
+}
+
 // Grouped plan, pass 1 of an order: per group g of destination j's sources
 // (entries sharing one BRDF row), the weighted shifted sum
 //   gsum[g, b, t] = sum_{e in g} scal[b, e] * prev[rowid[e], b, t - lag[e]]
@@ -303,11 +317,10 @@ __global__ __launch_bounds__(BS) void group_gather_kernel(
     const int32_t *__restrict__ rowid, const int32_t *__restrict__ lag,
     const double *__restrict__ scal, const float *__restrict__ prev,
     float *__restrict__ gsum) {
-  const int64_t tile = xcd_tile(blockIdx.x, nblocks);
-  const int tb = (int)(tile % n_tblk);
-  const int64_t jb = tile / n_tblk;
-  const int b = (int)(jb % B);
-  const int64_t j = jb / B;
+  int64_t j;
+  int b, tb;
+  dest_major_tile(xcd_tile(blockIdx.x, nblocks), nblocks / (B * n_tblk),
+                  n_tblk, j, b, tb);
   const int t0 = tb * (BS * R);
   const int t_end = min(T, t0 + BS * R);
   const int64_t e_begin = row_ptr[j], e_end = row_ptr[j + 1];
@@ -368,11 +381,10 @@ __global__ __launch_bounds__(BS) void group_spread_kernel(
     const double *__restrict__ rtab, const float *__restrict__ gsum,
     float *__restrict__ cur, float *__restrict__ tot_hi,
     float *__restrict__ tot_lo) {
-  const int64_t tile = xcd_tile(blockIdx.x, nblocks);
-  const int tb = (int)(tile % n_tblk);
-  const int64_t jb = tile / n_tblk;
-  const int b = (int)(jb % B);
-  const int64_t j = jb / B;
+  int64_t j;
+  int b, tb;
+  dest_major_tile(xcd_tile(blockIdx.x, nblocks), nblocks / (B * n_tblk),
+                  n_tblk, j, b, tb);
   const int t0 = tb * (BS * R);
   double acc[DP][R];
 #pragma unroll
@@ -436,27 +448,34 @@ __global__ void finalize_kernel(int64_t n, const float *__restrict__ hi,
 constexpr int kGatherR = 2;
 constexpr int kGatherC = 8;
 
-template <int DP, int R>
-int launch_grouped(int64_t P, int64_t D, int64_t B, int64_t T, int64_t Vd,
-                   const int64_t *row_ptr, const int64_t *grp_ptr,
-                   const int32_t *grp_key, const int32_t *rowid,
-                   const int32_t *lag, const double *scal, const double *rtab,
-                   float *gsum, const float *prev, float *cur, float *tot_hi,
-                   float *tot_lo, hipStream_t s) {
+int launch_gather(int64_t P, int64_t B, int64_t T, int64_t Vd,
+                  const int64_t *row_ptr, const int64_t *grp_ptr,
+                  const int32_t *rowid, const int32_t *lag,
+                  const double *scal, const float *prev, float *gsum,
+                  hipStream_t s) {
   constexpr int BS = 256;
-  const int64_t nt1 = ceil_div(T, (int64_t)BS * kGatherR);
-  const int64_t nb1 = P * B * nt1;
-  const int64_t nt2 = ceil_div(T, (int64_t)BS * R);
-  const int64_t nb2 = P * B * nt2;
-  SPR_REQUIRE(nb1 < (int64_t)0x7fffffff && nb2 < (int64_t)0x7fffffff,
-              "exchange grid too large");
+  const int64_t nt = ceil_div(T, (int64_t)BS * kGatherR);
+  const int64_t nb = P * B * nt;
+  SPR_REQUIRE(nb < (int64_t)0x7fffffff, "exchange grid too large");
   hipLaunchKernelGGL((group_gather_kernel<kGatherR, BS, kGatherC>),
-                     dim3((unsigned)nb1), dim3(BS), 0, s, (int)B, (int)T,
-                     (int)nt1, nb1, Vd, row_ptr, grp_ptr, rowid, lag, scal,
+                     dim3((unsigned)nb), dim3(BS), 0, s, (int)B, (int)T,
+                     (int)nt, nb, Vd, row_ptr, grp_ptr, rowid, lag, scal,
                      prev, gsum);
   SPR_LAUNCH_CHECK();
-  hipLaunchKernelGGL((group_spread_kernel<DP, R, BS>), dim3((unsigned)nb2),
-                     dim3(BS), 0, s, (int)D, (int)B, (int)T, (int)nt2, nb2,
+  return SPR_OK;
+}
+
+template <int DP, int R>
+int launch_spread(int64_t P, int64_t D, int64_t B, int64_t T,
+                  const int64_t *grp_ptr, const int32_t *grp_key,
+                  const double *rtab, const float *gsum, float *cur,
+                  float *tot_hi, float *tot_lo, hipStream_t s) {
+  constexpr int BS = 256;
+  const int64_t nt = ceil_div(T, (int64_t)BS * R);
+  const int64_t nb = P * B * nt;
+  SPR_REQUIRE(nb < (int64_t)0x7fffffff, "exchange grid too large");
+  hipLaunchKernelGGL((group_spread_kernel<DP, R, BS>), dim3((unsigned)nb),
+                     dim3(BS), 0, s, (int)D, (int)B, (int)T, (int)nt, nb,
                      grp_ptr, grp_key, rtab, gsum, cur, tot_hi, tot_lo);
   SPR_LAUNCH_CHECK();
   return SPR_OK;
@@ -823,37 +842,58 @@ extern "C" int spr_plan_grouped_fill(
   return SPR_OK;
 }
 
+extern "C" int spr_exchange_gather_grouped(
+    int64_t P, int64_t B, int64_t T, int64_t Vd, const int64_t *row_ptr,
+    const int64_t *grp_ptr, const int32_t *rowid, const int32_t *lag,
+    const double *scal, const float *prev, float *gsum, void *stream) {
+  SPR_REQUIRE(P > 0 && B > 0 && T > 0 && T < 0x3fffffff && Vd >= 0,
+              "spr_exchange_gather_grouped: bad sizes");
+  SPR_REQUIRE(row_ptr && grp_ptr && rowid && lag && scal && prev && gsum,
+              "spr_exchange_gather_grouped: null pointer");
+  return launch_gather(P, B, T, Vd, row_ptr, grp_ptr, rowid, lag, scal, prev,
+                       gsum, as_stream(stream));
+}
+
+extern "C" int spr_exchange_spread_grouped(
+    int64_t P, int64_t D, int64_t B, int64_t T, const int64_t *grp_ptr,
+    const int32_t *grp_key, const double *rtab, const float *gsum,
+    float *cur, float *tot_hi, float *tot_lo, void *stream) {
+  const int64_t DP = spr_padded_directions(D);
+  SPR_REQUIRE(DP > 0, "spr_exchange_spread_grouped: D=%lld unsupported",
+              (long long)D);
+  SPR_REQUIRE(P > 0 && B > 0 && T > 0 && T < 0x3fffffff,
+              "spr_exchange_spread_grouped: bad sizes");
+  SPR_REQUIRE(grp_ptr && grp_key && rtab && gsum && cur && tot_hi && tot_lo,
+              "spr_exchange_spread_grouped: null pointer");
+  hipStream_t s = as_stream(stream);
+#define SPR_SPREAD(DPV, RV)                                                   \
+  return launch_spread<DPV, RV>(P, D, B, T, grp_ptr, grp_key, rtab, gsum,     \
+                                cur, tot_hi, tot_lo, s)
+  switch (DP) {
+    case 1: SPR_SPREAD(1, 8);
+    case 2: SPR_SPREAD(2, 8);
+    case 4: SPR_SPREAD(4, 8);
+    case 8: SPR_SPREAD(8, 8);
+    case 16: SPR_SPREAD(16, 4);
+    case 32: SPR_SPREAD(32, 2);
+    case 64: SPR_SPREAD(64, 1);
+  }
+#undef SPR_SPREAD
+  return fail(SPR_ERR_UNSUPPORTED, "spr_exchange_spread_grouped: bad D");
+}
+
 extern "C" int spr_exchange_step_grouped(
     int64_t P, int64_t D, int64_t B, int64_t T, int64_t Vd,
     const int64_t *row_ptr, const int64_t *grp_ptr, const int32_t *grp_key,
     const int32_t *rowid, const int32_t *lag, const double *scal,
     const double *rtab, float *gsum, const float *prev, float *cur,
     float *tot_hi, float *tot_lo, void *stream) {
-  const int64_t DP = spr_padded_directions(D);
-  SPR_REQUIRE(DP > 0, "spr_exchange_step_grouped: D=%lld unsupported",
-              (long long)D);
-  SPR_REQUIRE(P > 0 && B > 0 && T > 0 && T < 0x3fffffff && Vd >= 0,
-              "spr_exchange_step_grouped: bad sizes");
-  SPR_REQUIRE(row_ptr && grp_ptr && grp_key && rowid && lag && scal && rtab &&
-                  gsum && prev && cur && tot_hi && tot_lo,
-              "spr_exchange_step_grouped: null pointer");
   SPR_REQUIRE(prev != cur, "spr_exchange_step_grouped: prev == cur");
-  hipStream_t s = as_stream(stream);
-#define SPR_GSTEP(DPV, RV)                                                    \
-  return launch_grouped<DPV, RV>(P, D, B, T, Vd, row_ptr, grp_ptr, grp_key,   \
-                                 rowid, lag, scal, rtab, gsum, prev, cur,     \
-                                 tot_hi, tot_lo, s)
-  switch (DP) {
-    case 1: SPR_GSTEP(1, 8);
-    case 2: SPR_GSTEP(2, 8);
-    case 4: SPR_GSTEP(4, 8);
-    case 8: SPR_GSTEP(8, 8);
-    case 16: SPR_GSTEP(16, 4);
-    case 32: SPR_GSTEP(32, 2);
-    case 64: SPR_GSTEP(64, 1);
-  }
-#undef SPR_GSTEP
-  return fail(SPR_ERR_UNSUPPORTED, "spr_exchange_step_grouped: bad D");
+  int rc = spr_exchange_gather_grouped(P, B, T, Vd, row_ptr, grp_ptr, rowid,
+                                       lag, scal, prev, gsum, stream);
+  if (rc) return rc;
+  return spr_exchange_spread_grouped(P, D, B, T, grp_ptr, grp_key, rtab,
+                                     gsum, cur, tot_hi, tot_lo, stream);
 }
 
 namespace {

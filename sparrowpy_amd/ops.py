@@ -470,22 +470,38 @@ class ExchangeState:
         prev and cur."""
         P, D, B, T = self.shape
         if isinstance(plan, GroupedPlan):
-            G = plan.n_groups
-            if self.gsum is None or self.gsum.shape[0] < max(G, 1):
-                self.gsum = torch.empty((max(G, 1), B, T),
-                                        dtype=torch.float32,
-                                        device=self.prev.device)
-            call("spr_exchange_step_grouped", P, D, B, T, plan.Vd,
-                 ptr(plan.row_ptr), ptr(plan.grp_ptr), ptr(plan.grp_key),
-                 ptr(plan.rowid), ptr(plan.lag), ptr(plan.scal),
-                 ptr(plan.rtab), ptr(self.gsum), ptr(self.prev),
-                 ptr(self.cur), ptr(self.hi), ptr(self.lo), stream_ptr())
+            self.gather(plan)
+            self.spread(plan)
         else:
             call("spr_exchange_step", P, D, B, T, ptr(plan.row_ptr),
                  ptr(plan.src), ptr(plan.lag), ptr(plan.dir), ptr(plan.coef),
                  ptr(self.prev), ptr(self.cur), ptr(self.hi), ptr(self.lo),
                  stream_ptr())
         self.prev, self.cur = self.cur, self.prev
+
+    def _gsum(self, plan):
+        P, D, B, T = self.shape
+        G = plan.n_groups
+        if self.gsum is None or self.gsum.shape[0] < max(G, 1):
+            self.gsum = torch.empty((max(G, 1), B, T), dtype=torch.float32,
+                                    device=self.prev.device)
+        return self.gsum
+
+    def gather(self, plan):
+        """Grouped plan, pass 1 of an order (per-group sums)."""
+        P, D, B, T = self.shape
+        call("spr_exchange_gather_grouped", P, B, T, plan.Vd,
+             ptr(plan.row_ptr), ptr(plan.grp_ptr), ptr(plan.rowid),
+             ptr(plan.lag), ptr(plan.scal), ptr(self.prev),
+             ptr(self._gsum(plan)), stream_ptr())
+
+    def spread(self, plan):
+        """Grouped plan, pass 2 of an order (directions and totals); does
+        not swap prev and cur."""
+        P, D, B, T = self.shape
+        call("spr_exchange_spread_grouped", P, D, B, T, ptr(plan.grp_ptr),
+             ptr(plan.grp_key), ptr(plan.rtab), ptr(self._gsum(plan)),
+             ptr(self.cur), ptr(self.hi), ptr(self.lo), stream_ptr())
 
     def finalize(self, out=None):
         """float32 total (hi + lo); ``out`` may be ``self.hi``."""

@@ -157,11 +157,15 @@ def street_canyon(seed=3, length=100., width=20., height=15.,
     L, Wd, H, dep = length, width, height, recess_depth
     holes = {0: [], 1: []}
     tries = 0
-    while sum(len(v) for v in holes.values()) < n_recesses and tries < 10000:
+    w_max, h_max = int(min(4, L - 2)), int(min(4, H - 2))
+    while (sum(len(v) for v in holes.values()) < n_recesses
+           and tries < 10000 and w_max >= 2 and h_max >= 2):
         tries += 1
         side = int(rng.integers(0, 2))
-        w = float(rng.integers(2, 5))
-        h = float(rng.integers(2, 5))
+        w = float(rng.integers(2, w_max + 1))
+        h = float(rng.integers(2, h_max + 1))
+        if L - w <= 1 or H - h <= 1:
+            continue
         x0 = float(rng.integers(1, int(L - w)))
         z0 = float(rng.integers(1, int(H - h)))
         cand = (x0, x0 + w, z0, z0 + h)
