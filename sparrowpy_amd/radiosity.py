@@ -51,11 +51,57 @@ def _to_numpy(t):
     return t
 
 
+def _direction_sets(sets):
+    """Per-wall direction sets (None, or Coordinates / [n,3] arrays / None
+    per wall) -> object array of Coordinates (None where unset)."""
+    if sets is None:
+        return None
+    out = np.empty(len(sets), object)
+    for i, c in enumerate(sets):
+        if c is None or (isinstance(c, str) and c == "None"):
+            continue
+        out[i] = Coordinates.from_cartesian(_cartesian(c))
+    return out
+
+
+def _as_state_value(v):
+    return "None" if v is None else v
+
+
+def _state_equal(a, b):
+    """Deep equality of two state values (dicts, lists, arrays, scalars)."""
+    if isinstance(a, dict) or isinstance(b, dict):
+        return (isinstance(a, dict) and isinstance(b, dict)
+                and a.keys() == b.keys()
+                and all(_state_equal(a[k], b[k]) for k in a))
+    if isinstance(a, (list, tuple)) or isinstance(b, (list, tuple)):
+        return (isinstance(a, (list, tuple)) and isinstance(b, (list, tuple))
+                and len(a) == len(b)
+                and all(_state_equal(x, y) for x, y in zip(a, b)))
+    if isinstance(a, str) or isinstance(b, str):
+        return a == b
+    a, b = np.asarray(a), np.asarray(b)
+    return a.shape == b.shape and a.dtype.kind == b.dtype.kind and \
+        np.array_equal(a, b)
+
+
 class DirectionalRadiosityFast:
     """Directional radiosity (reference ``RadiosityFast.py:15``)."""
 
     def __init__(self, walls_points, walls_normal, walls_up_vector,
-                 patches_points, n_patches, patch_to_wall_ids):
+                 patches_points, n_patches, patch_to_wall_ids,
+                 visibility_matrix=None, visible_patches=None,
+                 form_factors=None, form_factors_tilde=None,
+                 frequencies=None, brdf=None, brdf_index=None,
+                 brdf_incoming_directions=None,
+                 brdf_outgoing_directions=None,
+                 patch_2_brdf_outgoing_index=None, air_attenuation=None,
+                 speed_of_sound=None, etc_time_resolution=None,
+                 etc_duration=None, distance_patches_to_source=None,
+                 energy_init_source=None, energy_exchange_etc=None):
+        """Reference ``RadiosityFast.py:27-209``: geometry plus, optionally,
+        a saved state (the keys of :meth:`to_dict`).  Saved bake, source
+        and histogram arrays are uploaded to the GPU."""
         self._walls_points = np.atleast_3d(np.asarray(walls_points, float))
         self._walls_normal = np.atleast_2d(np.asarray(walls_normal, float))
         self._walls_up_vector = np.atleast_2d(
@@ -64,20 +110,43 @@ class DirectionalRadiosityFast:
         self._n_patches = int(n_patches)
         self._patch_to_wall_ids = np.atleast_1d(
             np.array(patch_to_wall_ids, dtype=np.int64))
-        self._frequencies = None
-        self._brdf = None
-        self._brdf_index = None
-        self._brdf_incoming_directions = None
-        self._brdf_outgoing_directions = None
-        self._air_attenuation = None
-        self._speed_of_sound = None
-        self._etc_time_resolution = None
-        self._etc_duration = None
+        self._frequencies = None if frequencies is None else \
+            np.atleast_1d(np.asarray(frequencies, float))
+        self._brdf = None if brdf is None else \
+            [np.asarray(b, float) for b in brdf]
+        self._brdf_index = None if brdf_index is None else \
+            np.asarray(brdf_index, dtype=np.int64)
+        self._brdf_incoming_directions = _direction_sets(
+            brdf_incoming_directions)
+        self._brdf_outgoing_directions = _direction_sets(
+            brdf_outgoing_directions)
+        self._air_attenuation = None if air_attenuation is None else \
+            np.atleast_1d(np.asarray(air_attenuation, float))
+        self._speed_of_sound = None if speed_of_sound is None else \
+            float(speed_of_sound)
+        self._etc_time_resolution = None if etc_time_resolution is None \
+            else float(etc_time_resolution)
+        self._etc_duration = None if etc_duration is None else \
+            float(etc_duration)
         self._source = None
         self._dev = {}
         self._bake = None
+        self._plan = None
         self._etc = None
         self.check()
+        if visibility_matrix is not None:
+            self._restore_bake(visibility_matrix, visible_patches,
+                               form_factors, form_factors_tilde,
+                               patch_2_brdf_outgoing_index)
+        if energy_init_source is not None:
+            self._dev["energy_init_source"] = ops.as_dev(
+                np.asarray(energy_init_source, float), torch.float64)
+            self._dev["distance_patches_to_source"] = ops.as_dev(
+                np.asarray(distance_patches_to_source, float),
+                torch.float64)
+        if energy_exchange_etc is not None:
+            self._etc = ops.as_dev(np.asarray(energy_exchange_etc),
+                                   torch.float32)
 
     # -- construction ---------------------------------------------------
     @classmethod
@@ -114,6 +183,150 @@ class DirectionalRadiosityFast:
             raise ValueError(
                 "patch_to_wall_ids does contain other ids than "
                 "range(n_walls)")
+
+    def _restore_bake(self, vis, pairs, ff, fft_dense, p2o):
+        """Upload a saved bake (the reference's visibility matrix, visible
+        pair list, form factors, direction map and, optionally, F~)."""
+        P = self.n_patches
+        vis = np.asarray(vis, dtype=bool)
+        if vis.shape != (P, P):
+            raise ValueError("visibility_matrix must be (n_patches, "
+                             "n_patches)")
+        if self._brdf_outgoing_directions is not None:
+            brdf, bidx, sources, receivers = self._brdf_tables()
+            D = receivers.shape[1]
+        else:
+            brdf = bidx = sources = receivers = None
+            D = 1
+        self._bake = dict(
+            vis=ops.as_dev(vis, torch.uint8),
+            pairs=ops.as_dev(np.asarray(pairs, dtype=np.int32).reshape(-1, 2)
+                             if pairs is not None else
+                             np.argwhere(vis).astype(np.int32), torch.int32),
+            ff=ops.as_dev(np.asarray(ff, float), torch.float64),
+            p2o=ops.as_dev(np.asarray(p2o, dtype=np.int64) if p2o is not None
+                           else np.zeros((P, P), np.int64), torch.int64),
+            D=D, n_bins=1 if self._frequencies is None else self.n_bins,
+            air=None if self._air_attenuation is None else
+            self._air_attenuation.copy(),
+            brdf=brdf, brdf_index=bidx, sources=sources, receivers=receivers,
+            fft_dense=None if fft_dense is None else
+            np.asarray(fft_dense, float))
+
+    # -- state (reference RadiosityFast.py:828-886) ------------------------
+    # Dense F~ [P,P,D,B] float64 is written only up to this size; larger
+    # states store "None" for it (it is rebuilt exactly from the bake).
+    MAX_DENSE_FFT_BYTES = 1 << 32
+
+    def to_dict(self) -> dict:
+        """The state under the reference's keys.
+
+        Arrays stay numpy arrays (the reference converts them to lists for
+        pyfar's writer); per-wall direction sets are lists of [n, 3] arrays
+        (None for walls without a BRDF); ``energy_exchange_etc`` is the
+        float32 total histogram; unset values are the string "None".
+        """
+        def dirs(sets):
+            if sets is None:
+                return None
+            return [None if c is None else np.asarray(c.cartesian)
+                    for c in sets]
+
+        fft = None
+        if self._bake is not None:
+            b = self._bake
+            P = self.n_patches
+            if b.get("fft_dense") is not None:
+                fft = b["fft_dense"]
+            elif P * P * b["D"] * b["n_bins"] * 8 <= self.MAX_DENSE_FFT_BYTES:
+                fft = self._form_factors_tilde
+        out = {
+            "walls_points": self._walls_points,
+            "walls_normal": self._walls_normal,
+            "walls_up_vector": self._walls_up_vector,
+            "patches_points": self._patches_points,
+            "n_patches": self._n_patches,
+            "patch_to_wall_ids": self._patch_to_wall_ids,
+            "visibility_matrix": self._visibility_matrix,
+            "visible_patches": self._visible_patches,
+            "form_factors": self._form_factors,
+            "form_factors_tilde": fft,
+            "frequencies": self._frequencies,
+            "brdf": self._brdf,
+            "brdf_index": self._brdf_index,
+            "brdf_incoming_directions": dirs(self._brdf_incoming_directions),
+            "brdf_outgoing_directions": dirs(self._brdf_outgoing_directions),
+            "patch_2_brdf_outgoing_index": self._patch_2_brdf_outgoing_index,
+            "air_attenuation": self._air_attenuation,
+            "speed_of_sound": self._speed_of_sound,
+            "etc_time_resolution": self._etc_time_resolution,
+            "etc_duration": self._etc_duration,
+            "distance_patches_to_source": self._distance_patches_to_source,
+            "energy_init_source": self._energy_init_source,
+            "energy_exchange_etc": self._energy_exchange_etc,
+        }
+        return {k: _as_state_value(v) for k, v in out.items()}
+
+    @classmethod
+    def from_dict(cls, input_dict: dict):
+        """Inverse of :meth:`to_dict` (reference ``RadiosityFast.py:880``);
+        the string "None" means unset."""
+        kw = {k: (None if isinstance(v, str) and v == "None" else v)
+              for k, v in input_dict.items()}
+        return cls(**kw)
+
+    def __eq__(self, other):
+        """Equal states (reference ``RadiosityFast.py:871-876``)."""
+        if not isinstance(other, DirectionalRadiosityFast):
+            return False
+        return _state_equal(self.to_dict(), other.to_dict())
+
+    __hash__ = None
+
+    def write(self, filename, compress=True):
+        """Write the state to a numpy ``.npz`` archive (the reference writes
+        a pyfar ``.far`` file with the same keys, ``RadiosityFast.py:828``).
+
+        Lists (``brdf``, direction sets) are stored element-wise as
+        ``key/<i>`` with ``key/len``; nothing is pickled.
+        """
+        flat = {}
+        for key, value in self.to_dict().items():
+            if isinstance(value, list):
+                flat[f"{key}/len"] = np.array(len(value))
+                for i, v in enumerate(value):
+                    flat[f"{key}/{i}"] = np.array("None") if v is None \
+                        else np.asarray(v)
+            else:
+                flat[key] = np.asarray(value)
+        save = np.savez_compressed if compress else np.savez
+        with open(filename, "wb") as f:
+            save(f, **flat)
+
+    @classmethod
+    def from_read(cls, filename):
+        """Read a state written by :meth:`write`."""
+        with np.load(filename, allow_pickle=False) as z:
+            data = {k: z[k] for k in z.files}
+        out = {}
+        for key in [k for k in data if "/" not in k]:
+            v = data[key]
+            out[key] = str(v) if v.dtype.kind == "U" else v
+        lists = {k.split("/")[0] for k in data if k.endswith("/len")}
+        for key in lists:
+            n = int(data[f"{key}/len"])
+            items = []
+            for i in range(n):
+                v = data[f"{key}/{i}"]
+                items.append(None if v.dtype.kind == "U" else v)
+            out[key] = items
+        for key in ("n_patches",):
+            out[key] = int(out[key])
+        for key in ("speed_of_sound", "etc_time_resolution",
+                    "etc_duration"):
+            if not isinstance(out[key], str):
+                out[key] = float(out[key])
+        return cls.from_dict(out)
 
     # -- materials --------------------------------------------------------
     def set_wall_brdf(self, wall_indexes, brdf, incoming_directions,
@@ -212,7 +425,8 @@ class DirectionalRadiosityFast:
             vis=vis, pairs=pairs, ff=ff, p2o=p2o, D=D, n_bins=n_bins,
             air=None if self._air_attenuation is None else
             self._air_attenuation.copy(),
-            brdf=brdf, brdf_index=bidx, sources=sources, receivers=receivers)
+            brdf=brdf, brdf_index=bidx, sources=sources, receivers=receivers,
+            fft_dense=None)
         self._plan = None
 
     # -- source -------------------------------------------------------------
@@ -274,6 +488,18 @@ class DirectionalRadiosityFast:
                 b["brdf_index"], b["D"], b["n_bins"], speed_of_sound,
                 time_resolution)
         plan = None
+        if b.get("fft_dense") is not None:
+            # a saved F~ is used as given (reference: the attribute read
+            # by _energy_exchange)
+            ctr = self.patches_center
+            d = ctr[:, None, :] - ctr[None, :, :]
+            dist = np.sqrt((d[..., 0] * d[..., 0] + d[..., 1] * d[..., 1])
+                           + d[..., 2] * d[..., 2])
+            plan = ops.ExchangePlan.from_dense(
+                b["vis"], dist, b["p2o"], b["fft_dense"], speed_of_sound,
+                time_resolution)
+            self._plan = (key, plan)
+            return plan
         if kind in (None, "grouped"):
             try:
                 plan = ops.GroupedPlan.from_geometry(*args)
@@ -410,6 +636,8 @@ class DirectionalRadiosityFast:
         if self._bake is None:
             return None
         b = self._bake
+        if b.get("fft_dense") is not None:
+            return b["fft_dense"]
         g = self._geometry()
         return _to_numpy(ops.form_factors_with_directivity_dim(
             b["vis"], b["ff"], b["n_bins"], g["centers"], g["areas"],

@@ -1,0 +1,145 @@
+"""State format: ``to_dict`` / ``from_dict`` / ``write`` / ``from_read`` /
+``__eq__`` of the drop-in class, restating the reference's
+tests/test_DirectionalRadiosityFast.py:21-120 (test_io,
+test_io_within_simulation).  The stages before the bake run on the CPU;
+the rest needs the GPU.
+"""
+import numpy as np
+import pytest
+
+from conftest import rel_err_within_db
+
+from sparrowpy_amd import brdf as B
+from sparrowpy_amd import testing
+from sparrowpy_amd.coordinates import Coordinates, FrequencyData
+from sparrowpy_amd.radiosity import DirectionalRadiosityFast
+
+REFERENCE_KEYS = [
+    "walls_points", "walls_normal", "walls_up_vector", "patches_points",
+    "n_patches", "patch_to_wall_ids", "visibility_matrix", "visible_patches",
+    "form_factors", "form_factors_tilde", "frequencies", "brdf",
+    "brdf_index", "brdf_incoming_directions", "brdf_outgoing_directions",
+    "patch_2_brdf_outgoing_index", "air_attenuation", "speed_of_sound",
+    "etc_time_resolution", "etc_duration", "distance_patches_to_source",
+    "energy_init_source", "energy_exchange_etc"]
+
+
+def _roundtrip(r, path):
+    r.write(path)
+    return DirectionalRadiosityFast.from_read(path)
+
+
+def _materials(r, n_walls, apply_brdf, apply_attenuation):
+    freqs = [1000.]
+    if apply_brdf:
+        r.set_wall_brdf(np.arange(n_walls),
+                        FrequencyData(np.ones_like(freqs), freqs),
+                        Coordinates(0, 0, 1, weights=1),
+                        Coordinates(0, 0, 1, weights=1))
+    if apply_attenuation:
+        r.set_air_attenuation(FrequencyData(np.ones_like(freqs), freqs))
+
+
+def test_keys_match_reference():
+    walls = testing.shoebox_room_stub(1, 1, 1)[:2]
+    r = DirectionalRadiosityFast.from_polygon(walls, 1)
+    d = r.to_dict()
+    assert list(d) == REFERENCE_KEYS
+    assert d["visibility_matrix"] == "None"
+    assert d["energy_exchange_etc"] == "None"
+
+
+@pytest.mark.parametrize("apply_brdf", [True, False])
+@pytest.mark.parametrize("apply_attenuation", [True, False])
+def test_io_before_bake(apply_brdf, apply_attenuation, tmp_path):
+    walls = testing.shoebox_room_stub(1, 1, 1)[:2]
+    r = DirectionalRadiosityFast.from_polygon(walls, 1)
+    assert _roundtrip(r, tmp_path / "a.npz") == r
+    _materials(r, len(walls), apply_brdf, apply_attenuation)
+    out = _roundtrip(r, tmp_path / "b.npz")
+    assert out == r
+    # a different state is not equal
+    other = DirectionalRadiosityFast.from_polygon(walls, 0.5)
+    assert other != r
+
+
+def test_io_partial_brdf(tmp_path):
+    """A BRDF on a subset of walls leaves the other direction sets unset."""
+    walls = testing.shoebox_room_stub(2, 2, 2)
+    r = DirectionalRadiosityFast.from_polygon(walls, 1)
+    c = Coordinates(0, 0, 1, weights=1)
+    br = B.create_from_scattering(c, c, FrequencyData([1.], [500.]),
+                                  FrequencyData([.2], [500.]))
+    r.set_wall_brdf([0, 3], br, c, c)
+    d = r.to_dict()
+    assert d["brdf_incoming_directions"][1] is None
+    out = _roundtrip(r, tmp_path / "p.npz")
+    assert out == r
+    np.testing.assert_array_equal(out._brdf_index, r._brdf_index)
+
+
+def test_write_uncompressed_and_no_pickle(tmp_path):
+    walls = testing.shoebox_room_stub(1, 1, 1)
+    r = DirectionalRadiosityFast.from_polygon(walls, 1)
+    _materials(r, len(walls), True, True)
+    r.write(tmp_path / "u.npz", compress=False)
+    with np.load(tmp_path / "u.npz", allow_pickle=False) as z:
+        assert all(z[k].dtype != object for k in z.files)
+    assert DirectionalRadiosityFast.from_read(tmp_path / "u.npz") == r
+
+
+@pytest.fixture(scope="module")
+def gpu():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("apply_brdf", [True, False])
+@pytest.mark.parametrize("apply_attenuation", [True, False])
+def test_io_all_stages(gpu, apply_brdf, apply_attenuation, tmp_path):
+    walls = testing.shoebox_room_stub(1, 1, 1)[:2]
+    r = DirectionalRadiosityFast.from_polygon(walls, 1)
+    _materials(r, len(walls), apply_brdf, apply_attenuation)
+    r.bake_geometry()
+    assert _roundtrip(r, tmp_path / "1.npz") == r
+    r.init_source_energy(Coordinates(.5, .5, .5))
+    assert _roundtrip(r, tmp_path / "2.npz") == r
+    r.calculate_energy_exchange(343, 1 / 1000, 1, 3)
+    out = _roundtrip(r, tmp_path / "3.npz")
+    assert out == r
+    assert out._energy_exchange_etc.shape == r._energy_exchange_etc.shape
+
+
+@pytest.mark.gpu
+def test_io_within_simulation(gpu, tmp_path):
+    """Continue a simulation from a saved state at each stage; results are
+    those of the uninterrupted run."""
+    walls = testing.shoebox_room_stub(1, 1, 1)
+    r = DirectionalRadiosityFast.from_polygon(walls, 1)
+    c = Coordinates(0, 0, 1, weights=1)
+    freqs = np.array([1000.])
+    br = B.create_from_scattering(c, c, FrequencyData(1, freqs),
+                                  FrequencyData(.1, freqs))
+    r.set_wall_brdf(np.arange(len(walls)), br, c, c)
+    r.set_air_attenuation(FrequencyData(np.zeros_like(freqs), freqs))
+    rr = _roundtrip(r, tmp_path / "s.npz")
+    rr.bake_geometry()
+    r.bake_geometry()
+    rr = _roundtrip(r, tmp_path / "s.npz")
+    r.init_source_energy(Coordinates(.5, .5, .5))
+    rr.init_source_energy(Coordinates(.5, .5, .5))
+    rr = _roundtrip(r, tmp_path / "s.npz")
+    r.calculate_energy_exchange(343, 1 / 1000, 1, 3, recalculate=True)
+    rr.calculate_energy_exchange(343, 1 / 1000, 1, 3, recalculate=True)
+    # the restored object runs the exchange on the saved dense F~, the
+    # original on its pair table: same terms, same order per bin
+    a, b = rr.to_dict(), r.to_dict()
+    etc_a, etc_b = a.pop("energy_exchange_etc"), b.pop("energy_exchange_etc")
+    assert DirectionalRadiosityFast.from_dict(a) == \
+        DirectionalRadiosityFast.from_dict(b)
+    assert rel_err_within_db(etc_a, etc_b) <= 1e-6
+    rec = Coordinates(.3, .4, .6)
+    assert rel_err_within_db(rr.collect_energy_receiver_mono(rec).time,
+                             r.collect_energy_receiver_mono(rec).time) <= 1e-6
