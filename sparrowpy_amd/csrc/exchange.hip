@@ -305,11 +305,78 @@ __device__ __forceinline__ void dest_major_tile(int64_t tile, int64_t P,
   b = (int)(rest / n_tblk);
 }
 
+// Raw buffer access with a gfx9 buffer descriptor: a load whose byte
+// offset lies outside [0, bytes) returns 0 and a store there is dropped,
+// with no branch and no memory access.  This lets every lane issue its
+// loads unconditionally, so the compiler can count them and keep the next
+// chunk's loads in flight while the current chunk is summed.
+constexpr int kBufferWord3 = 0x00020000;   // gfx9 descriptor word 3
+constexpr uint32_t kNoOffset = 0x80000000u;  // beyond every range used here
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void *p,
+                                                            uint32_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), 0,
+                                           (int)bytes, kBufferWord3);
+}
+
+__device__ __forceinline__ float buffer_load(__amdgpu_buffer_rsrc_t r,
+                                             uint32_t off) {
+  return __builtin_bit_cast(
+      float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)off, 0, 0));
+}
+
+__device__ __forceinline__ void buffer_store(float v,
+                                             __amdgpu_buffer_rsrc_t r,
+                                             uint32_t off) {
+  __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r,
+                                        (int)off, 0, 0);
+}
+
 // Grouped plan, pass 1 of an order: per group g of destination j's sources
 // (entries sharing one BRDF row), the weighted shifted sum
 //   gsum[g, b, t] = sum_{e in g} scal[b, e] * prev[rowid[e], b, t - lag[e]]
-// in float64, stored as float32.  Only two accumulators per lane: the
-// kernel is a pure gather with many loads in flight.
+// in float64, stored as float32.
+//
+// The destination's entries are walked in blocks of 64: one coalesced
+// vector load brings a block's metadata (source row, delay word, weight),
+// one entry per lane, and each entry's values are broadcast with
+// v_readlane.  A block is fully unrolled into 64/C chunks of C entries;
+// chunk q+1's C*R gathers are issued into one register set while chunk q
+// is summed from the other, so no register holding an in-flight load is
+// ever copied and the compiler can wait for exactly the chunk it needs.
+// Each lane owns R bins.  A source bin before the histogram start
+// (t < lag) is an out-of-range buffer offset and reads as 0; so does
+// every bin of an entry past the row end or whose delay reaches beyond
+// the tile (descriptor of size 0).  Requires T < 2^29, so that
+// t - lag >= -T maps to a byte offset >= 2^31.
+constexpr int kMetaBlock = 64;
+
+struct EntryMeta {
+  int32_t row, lag;
+  double w;
+};
+
+__device__ __forceinline__ EntryMeta load_meta(
+    int64_t e0, int64_t e_end, const int32_t *__restrict__ rowid,
+    const int32_t *__restrict__ lag, const double *__restrict__ w) {
+  const int64_t e = e0 + (int64_t)(threadIdx.x & 63);
+  const bool in = e < e_end;
+  const int64_t ec = in ? e : e_end - 1;
+  EntryMeta m;
+  m.row = rowid[ec];
+  m.lag = in ? lag[ec] : kLagMask;  // past the row end: no bins, no flag
+  m.w = in ? w[ec] : 0.0;
+  return m;
+}
+
+__device__ __forceinline__ double readlane_f64(double v, int k) {
+  const int64_t u = __builtin_bit_cast(int64_t, v);
+  const int32_t lo = __builtin_amdgcn_readlane((int32_t)u, k);
+  const int32_t hi = __builtin_amdgcn_readlane((int32_t)(u >> 32), k);
+  return __builtin_bit_cast(double,
+                            ((int64_t)hi << 32) | (int64_t)(uint32_t)lo);
+}
+
 template <int R, int BS, int C>
 __global__ __launch_bounds__(BS) void group_gather_kernel(
     int B, int T, int n_tblk, int64_t nblocks, int64_t Vd,
@@ -317,6 +384,8 @@ __global__ __launch_bounds__(BS) void group_gather_kernel(
     const int32_t *__restrict__ rowid, const int32_t *__restrict__ lag,
     const double *__restrict__ scal, const float *__restrict__ prev,
     float *__restrict__ gsum) {
+  static_assert(kMetaBlock % (2 * C) == 0, "chunks pair up in a block");
+  constexpr int NQ = kMetaBlock / C;  // chunks per block
   int64_t j;
   int b, tb;
   dest_major_tile(xcd_tile(blockIdx.x, nblocks), nblocks / (B * n_tblk),
@@ -326,95 +395,196 @@ __global__ __launch_bounds__(BS) void group_gather_kernel(
   const int64_t e_begin = row_ptr[j], e_end = row_ptr[j + 1];
   if (e_end <= e_begin) return;
   const double *w = scal + (int64_t)b * Vd;
+  const uint32_t row_bytes = (uint32_t)T * 4u;
+  const int tl = t0 + (int)threadIdx.x;
   int64_t g = grp_ptr[j];
-  auto row_of = [&](int64_t ee) { return (int64_t)rowid[ee]; };
+
+  // gathers of chunk q (entries q*C .. q*C+C-1 of the block of `m`)
+  auto fetch = [&](const EntryMeta &m, int q, float (&v)[C][R]) {
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+      const int k = q * C + c;
+      const int32_t rk = __builtin_amdgcn_readlane(m.row, k);
+      const int32_t lk = __builtin_amdgcn_readlane(m.lag, k);
+      const int n = min(lk & kLagMask, T);
+      const float *src = prev + ((int64_t)rk * B + b) * T;
+      const __amdgpu_buffer_rsrc_t rs =
+          make_rsrc(src, n < t_end ? row_bytes : 0u);
+#pragma unroll
+      for (int r = 0; r < R; ++r)
+        v[c][r] = buffer_load(rs, (uint32_t)(tl + BS * r - n) * 4u);
+    }
+  };
+
   double S[R];
 #pragma unroll
   for (int r = 0; r < R; ++r) S[r] = 0.0;
-  float buf[C][R];
-  int n[C];
-  bool act[C];
-  gather_chunk<R, BS, C>(e_begin, e_end, B, T, b, t0, t_end, row_of, lag,
-                         prev, buf, n, act);
-  for (int64_t e = e_begin; e < e_end; e += C) {
-    float nbuf[C][R];
-    int nn[C];
-    bool nact[C];
-    gather_chunk<R, BS, C>(e + C, e_end, B, T, b, t0, t_end, row_of, lag,
-                           prev, nbuf, nn, nact);
+  // sum chunk q; store a group's sums at its last entry
+  auto consume = [&](const EntryMeta &m, int q, const float (&v)[C][R]) {
 #pragma unroll
     for (int c = 0; c < C; ++c) {
-      if (e + c >= e_end) continue;
-      if (act[c]) {
-        const double wc = w[e + c];
+      const int k = q * C + c;
+      const double wk = readlane_f64(m.w, k);
 #pragma unroll
-        for (int r = 0; r < R; ++r) S[r] = fma(wc, (double)buf[c][r], S[r]);
-      }
-      if (n[c] & kLastInGroup) {
-        float *o = gsum + (g * B + b) * (int64_t)T;
+      for (int r = 0; r < R; ++r) S[r] = fma(wk, (double)v[c][r], S[r]);
+      if (__builtin_amdgcn_readlane(m.lag, k) & kLastInGroup) {
+        const __amdgpu_buffer_rsrc_t os =
+            make_rsrc(gsum + (g * B + b) * (int64_t)T, row_bytes);
 #pragma unroll
         for (int r = 0; r < R; ++r) {
-          const int t = t0 + (int)threadIdx.x + BS * r;
-          if (t < T) o[t] = (float)S[r];
+          buffer_store((float)S[r], os, (uint32_t)(tl + BS * r) * 4u);
           S[r] = 0.0;
         }
         g += 1;
       }
     }
+  };
+
+  EntryMeta cur = load_meta(e_begin, e_end, rowid, lag, w);
+  EntryMeta next = load_meta(e_begin + kMetaBlock, e_end, rowid, lag, w);
+  float va[C][R], vb[C][R];
+  fetch(cur, 0, va);
+  for (int64_t blk = e_begin; blk < e_end; blk += kMetaBlock) {
 #pragma unroll
-    for (int c = 0; c < C; ++c) {
-      n[c] = nn[c];
-      act[c] = nact[c];
-#pragma unroll
-      for (int r = 0; r < R; ++r) buf[c][r] = nbuf[c][r];
+    for (int q = 0; q < NQ; q += 2) {
+      fetch(cur, q + 1, vb);
+      consume(cur, q, va);
+      if (q + 2 < NQ)
+        fetch(cur, q + 2, va);
+      else
+        fetch(next, 0, va);  // first chunk of the next block
+      consume(cur, q + 1, vb);
     }
+    cur = next;
+    next = load_meta(blk + 2 * kMetaBlock, e_end, rowid, lag, w);
   }
 }
 
 // Grouped plan, pass 2: spread the group sums over the D directions with
 // their BRDF rows and update the order histogram and the running total:
 //   cur[j, :, b, t] = sum_{g of j} rtab[key_g, b, :] * gsum[g, b, t].
-template <int DP, int R, int BS>
-__global__ __launch_bounds__(BS) void group_spread_kernel(
+// A workgroup of 4 waves takes one (destination, band, time block); each
+// lane keeps DW (up to 32) directions of R bins as float64 sums, so
+// DW * R = 32; beyond 32 directions the waves split them NWD = DP / DW ways
+// and the bins NWT = 4 / NWD ways.
+// The running total (hi + lo, updated in float64) is read K (d, r) pairs
+// at a time through buffer descriptors whose out-of-range offsets stand
+// for padded directions and bins past T (loads give 0, stores are
+// dropped).  Requires D * B * T < 2^29 (byte offsets below 2^31).
+template <int DP>
+struct SpreadShape {
+  static constexpr int DW = DP < 32 ? DP : 32;   // directions per lane
+  static constexpr int NWD = DP / DW;            // wave groups along d
+  static constexpr int NWT = 4 / NWD;            // wave groups along t
+  static constexpr int R = DW >= 32 ? 1 : 32 / DW > 8 ? 8 : 32 / DW;
+  static constexpr int TILE = NWT * 64 * R;      // bins per workgroup
+};
+
+template <int DP>
+__global__ __launch_bounds__(256) void group_spread_kernel(
     int D, int B, int T, int n_tblk, int64_t nblocks,
     const int64_t *__restrict__ grp_ptr, const int32_t *__restrict__ grp_key,
     const double *__restrict__ rtab, const float *__restrict__ gsum,
     float *__restrict__ cur, float *__restrict__ tot_hi,
     float *__restrict__ tot_lo) {
+  using Sh = SpreadShape<DP>;
+  constexpr int DW = Sh::DW, R = Sh::R, K = 8;
   int64_t j;
   int b, tb;
   dest_major_tile(xcd_tile(blockIdx.x, nblocks), nblocks / (B * n_tblk),
                   n_tblk, j, b, tb);
-  const int t0 = tb * (BS * R);
-  double acc[DP][R];
+  const int wave = (int)threadIdx.x >> 6, lane = (int)threadIdx.x & 63;
+  const int wd = wave % Sh::NWD, wt = wave / Sh::NWD;
+  const int d0 = wd * DW;  // first direction of this wave
+  const int tl = tb * Sh::TILE + wt * (64 * R) + lane;  // bin of r = 0
+  const uint32_t row_bytes = (uint32_t)T * 4u;
+
+  double acc[DW][R];
 #pragma unroll
-  for (int d = 0; d < DP; ++d)
+  for (int d = 0; d < DW; ++d)
 #pragma unroll
     for (int r = 0; r < R; ++r) acc[d][r] = 0.0;
+  // groups are taken GC at a time, the next GC rows' loads in flight while
+  // the current ones are summed; rows past the destination's last group
+  // read as 0 (descriptor of size 0) and add nothing
+  constexpr int GC = DW >= 32 ? 2 : 4;
   const int64_t g0 = grp_ptr[j], g1 = grp_ptr[j + 1];
-  float sv[R], nsv[R];
-  auto load = [&](int64_t g, float (&v)[R]) {
-    const float *p = gsum + (g * B + b) * (int64_t)T;
+  auto load = [&](int64_t g, float (&v)[GC][R]) {
 #pragma unroll
-    for (int r = 0; r < R; ++r) {
-      const int t = t0 + (int)threadIdx.x + BS * r;
-      v[r] = (g < g1 && t < T) ? p[t] : 0.f;
+    for (int c = 0; c < GC; ++c) {
+      const int64_t gg = g + c;
+      const __amdgpu_buffer_rsrc_t rs =
+          make_rsrc(gsum + (min(gg, g1 - 1) * B + b) * (int64_t)T,
+                    gg < g1 ? row_bytes : 0u);
+#pragma unroll
+      for (int r = 0; r < R; ++r)
+        v[c][r] = buffer_load(rs, (uint32_t)(tl + 64 * r) * 4u);
     }
   };
-  if (g1 > g0) load(g0, sv);
-  for (int64_t g = g0; g < g1; ++g) {
-    load(g + 1, nsv);
-    const double *rt = rtab + ((int64_t)grp_key[g] * B + b) * DP;
+  auto step = [&](int64_t g, const float (&v)[GC][R]) {
 #pragma unroll
-    for (int d = 0; d < DP; ++d) {
-      const double cd = rt[d];
+    for (int c = 0; c < GC; ++c) {
+      const int64_t gg = min(g + c, g1 - 1);
+      const double *rt = rtab + ((int64_t)grp_key[gg] * B + b) * DP + d0;
 #pragma unroll
-      for (int r = 0; r < R; ++r) acc[d][r] = fma(cd, (double)sv[r], acc[d][r]);
+      for (int d = 0; d < DW; ++d) {
+        const double cd = rt[d];
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+          acc[d][r] = fma(cd, (double)v[c][r], acc[d][r]);
+      }
+    }
+  };
+  if (g1 > g0) {
+    float va[GC][R], vb[GC][R];
+    load(g0, va);
+    for (int64_t g = g0; g < g1; g += 2 * GC) {
+      // the scheduling barriers keep each prefetch ahead of the sums
+      load(g + GC, vb);
+      __builtin_amdgcn_sched_barrier(0);
+      step(g, va);
+      load(g + 2 * GC, va);
+      __builtin_amdgcn_sched_barrier(0);
+      step(g + GC, vb);
+    }
+  }
+
+  // histogram and running total: element (j, d, b, t) at
+  // ((j * D + d) * B + b) * T + t; one descriptor spans d = 0 .. D-1
+  const int64_t dstride = (int64_t)B * T;
+  const int64_t base = j * D * dstride + (int64_t)b * T;
+  const uint32_t span = (uint32_t)(((int64_t)(D - 1) * dstride + T) * 4);
+  const __amdgpu_buffer_rsrc_t rc = make_rsrc(cur + base, span);
+  const __amdgpu_buffer_rsrc_t rh = make_rsrc(tot_hi + base, span);
+  const __amdgpu_buffer_rsrc_t rl = make_rsrc(tot_lo + base, span);
+  constexpr int NP = DW * R;  // (d, r) pairs of this lane
+#pragma unroll
+  for (int p0 = 0; p0 < NP; p0 += K) {
+    uint32_t off[K];
+    float h[K], l[K];
+#pragma unroll
+    for (int q = 0; q < K; ++q) {
+      const int p = p0 + q;
+      if (p >= NP) break;
+      const int d = d0 + p / R, r = p % R;
+      const int t = tl + 64 * r;
+      off[q] = (d < D && t < T) ? (uint32_t)((int64_t)d * dstride + t) * 4u
+                                : kNoOffset;
+      h[q] = buffer_load(rh, off[q]);
+      l[q] = buffer_load(rl, off[q]);
     }
 #pragma unroll
-    for (int r = 0; r < R; ++r) sv[r] = nsv[r];
+    for (int q = 0; q < K; ++q) {
+      const int p = p0 + q;
+      if (p >= NP) break;
+      const double a = acc[p / R][p % R];
+      const double sum = ((double)h[q] + (double)l[q]) + a;
+      const float hs = (float)sum;
+      buffer_store((float)a, rc, off[q]);
+      buffer_store(hs, rh, off[q]);
+      buffer_store((float)(sum - (double)hs), rl, off[q]);
+    }
   }
-  store_tile<DP, R, BS>(acc, D, B, T, j, b, t0, cur, tot_hi, tot_lo);
 }
 
 template <int DP, int R>
@@ -457,6 +627,7 @@ int launch_gather(int64_t P, int64_t B, int64_t T, int64_t Vd,
   const int64_t nt = ceil_div(T, (int64_t)BS * kGatherR);
   const int64_t nb = P * B * nt;
   SPR_REQUIRE(nb < (int64_t)0x7fffffff, "exchange grid too large");
+  SPR_REQUIRE(T < ((int64_t)1 << 29), "histogram too long (T >= 2^29)");
   hipLaunchKernelGGL((group_gather_kernel<kGatherR, BS, kGatherC>),
                      dim3((unsigned)nb), dim3(BS), 0, s, (int)B, (int)T,
                      (int)nt, nb, Vd, row_ptr, grp_ptr, rowid, lag, scal,
@@ -465,17 +636,18 @@ int launch_gather(int64_t P, int64_t B, int64_t T, int64_t Vd,
   return SPR_OK;
 }
 
-template <int DP, int R>
+template <int DP>
 int launch_spread(int64_t P, int64_t D, int64_t B, int64_t T,
                   const int64_t *grp_ptr, const int32_t *grp_key,
                   const double *rtab, const float *gsum, float *cur,
                   float *tot_hi, float *tot_lo, hipStream_t s) {
-  constexpr int BS = 256;
-  const int64_t nt = ceil_div(T, (int64_t)BS * R);
+  const int64_t nt = ceil_div(T, (int64_t)SpreadShape<DP>::TILE);
   const int64_t nb = P * B * nt;
   SPR_REQUIRE(nb < (int64_t)0x7fffffff, "exchange grid too large");
-  hipLaunchKernelGGL((group_spread_kernel<DP, R, BS>), dim3((unsigned)nb),
-                     dim3(BS), 0, s, (int)D, (int)B, (int)T, (int)nt, nb,
+  SPR_REQUIRE(D * B * T < ((int64_t)1 << 29),
+              "histogram too large for one patch (D*B*T >= 2^29)");
+  hipLaunchKernelGGL((group_spread_kernel<DP>), dim3((unsigned)nb),
+                     dim3(256), 0, s, (int)D, (int)B, (int)T, (int)nt, nb,
                      grp_ptr, grp_key, rtab, gsum, cur, tot_hi, tot_lo);
   SPR_LAUNCH_CHECK();
   return SPR_OK;
@@ -866,17 +1038,17 @@ extern "C" int spr_exchange_spread_grouped(
   SPR_REQUIRE(grp_ptr && grp_key && rtab && gsum && cur && tot_hi && tot_lo,
               "spr_exchange_spread_grouped: null pointer");
   hipStream_t s = as_stream(stream);
-#define SPR_SPREAD(DPV, RV)                                                   \
-  return launch_spread<DPV, RV>(P, D, B, T, grp_ptr, grp_key, rtab, gsum,     \
-                                cur, tot_hi, tot_lo, s)
+#define SPR_SPREAD(DPV)                                                       \
+  return launch_spread<DPV>(P, D, B, T, grp_ptr, grp_key, rtab, gsum, cur,    \
+                            tot_hi, tot_lo, s)
   switch (DP) {
-    case 1: SPR_SPREAD(1, 8);
-    case 2: SPR_SPREAD(2, 8);
-    case 4: SPR_SPREAD(4, 8);
-    case 8: SPR_SPREAD(8, 8);
-    case 16: SPR_SPREAD(16, 4);
-    case 32: SPR_SPREAD(32, 2);
-    case 64: SPR_SPREAD(64, 1);
+    case 1: SPR_SPREAD(1);
+    case 2: SPR_SPREAD(2);
+    case 4: SPR_SPREAD(4);
+    case 8: SPR_SPREAD(8);
+    case 16: SPR_SPREAD(16);
+    case 32: SPR_SPREAD(32);
+    case 64: SPR_SPREAD(64);
   }
 #undef SPR_SPREAD
   return fail(SPR_ERR_UNSUPPORTED, "spr_exchange_spread_grouped: bad D");
