@@ -118,6 +118,26 @@ def cpu_baseline(plan, D, T, seconds, seed=0):
     return upd / dt, nthreads, T_s, n_dest
 
 
+def measured_traffic(kernel, config):
+    """HBM bytes per launch of ``kernel`` from the committed rocprofv3
+    counter summary of this workload (FETCH_SIZE and WRITE_SIZE passes,
+    scaled by a copy of known size), or None if it does not cover it."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)),
+                        "profiles", "traffic_street_canyon.json")
+    try:
+        with open(path) as f:
+            prof = json.load(f)
+    except (OSError, ValueError):
+        return None
+    if any(prof["workload"].get(k) != config.get(k)
+           for k in prof["workload"]):
+        return None
+    for name, v in prof["kernels"].items():
+        if name.split("<")[0] == kernel:
+            return v["bytes"]
+    return None
+
+
 def copy_calibration(n_bytes=1 << 32, reps=3):
     """Copy n_bytes device-to-device ``reps`` times (one kernel each)."""
     import torch
@@ -296,6 +316,7 @@ def main(argv=None):
             "bound": "hbm", "achieved": achieved_gbs, "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": achieved_gbs / HBM_PEAK_GBS,
             "traffic": None,
+            "traffic_source": None,
             "kernel": dom, "kernel_ms": dom_s * 1e3,
             "algorithmic_bytes": dom_bytes,
         },
@@ -308,6 +329,13 @@ def main(argv=None):
         },
         "setup": t,
     }
+    traffic = measured_traffic(dom, line["config"])
+    if traffic is not None:
+        line["roofline"]["traffic"] = traffic
+        line["roofline"]["traffic_source"] = (
+            "profiles/exchange_street_canyon.md (rocprofv3 FETCH_SIZE x2 + "
+            "WRITE_SIZE x1, per launch)")
+        line["roofline"]["traffic_GB/s"] = traffic / dom_s / 1e9
     if not args.no_cpu_baseline:
         log("cpu baseline sample")
         ups, nth, T_s, n_dest = cpu_baseline(plan, D, T, args.cpu_seconds)
