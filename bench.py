@@ -59,6 +59,7 @@ def _generic_rows(plan, D):
     row_ptr = plan.row_ptr.cpu().numpy()
     lag = plan.lag.cpu().numpy()
     if isinstance(plan, ops.GroupedPlan):
+        flag = (lag & ops.GroupedPlan.LAST_IN_GROUP) != 0
         lag = lag & (ops.GroupedPlan.LAST_IN_GROUP - 1)
         rowid = plan.rowid.cpu().numpy().astype(np.int64)
         src, dirs = rowid // D, rowid % D
@@ -67,6 +68,13 @@ def _generic_rows(plan, D):
         g_of_e = np.searchsorted(gs, np.arange(rowid.size), side="right") - 1
         rt = plan.rtab[:, 0, :D].cpu().numpy()
         coef = plan.scal[0].cpu().numpy()[:, None] * rt[gk[g_of_e]]
+        if plan.diffuse_specular:
+            # entries after a row's last flagged entry have no specular part
+            row_of_e = np.repeat(np.arange(plan.P), np.diff(row_ptr))
+            after = np.cumsum(flag[::-1])[::-1]
+            after_end = np.append(after, 0)[row_ptr[1:]][row_of_e]
+            coef[(after - after_end) == 0] = 0.0
+            coef += plan.dscal[0].cpu().numpy()[:, None]
     else:
         src = plan.src.cpu().numpy()
         dirs = plan.dir.cpu().numpy()
@@ -163,7 +171,7 @@ def main(argv=None):
                     help="target CPU time of the oracle baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--plan", default="auto",
-                    choices=["auto", "grouped", "generic"],
+                    choices=["auto", "ds", "grouped", "generic"],
                     help="exchange plan (auto: grouped when sources share "
                          "BRDF rows)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -211,7 +219,8 @@ def main(argv=None):
     t0 = tic()
     plan = r._exchange_plan(c, dt_bin,
                             None if args.plan == "auto" else args.plan)
-    kind = "grouped" if isinstance(plan, ops.GroupedPlan) else "generic"
+    kind = ("generic" if not isinstance(plan, ops.GroupedPlan) else
+            "ds" if plan.diffuse_specular else "grouped")
     t["plan_s"] = tic() - t0
     log(f"plan {t['plan_s']:.2f} s ({kind}), directed pairs={plan.Vd}")
 
@@ -230,14 +239,15 @@ def main(argv=None):
     U = int((B * torch.clamp(T - lag, min=0)).sum().item())
     Vd = plan.Vd
     stream = torch.cuda.current_stream()
-    n_ev = 3 if kind == "grouped" else 2
+    two_pass = kind in ("ds", "grouped")
+    n_ev = 3 if two_pass else 2
     ev = [[torch.cuda.Event(enable_timing=True) for _ in range(n_ev)]
           for _ in range(args.steps)]
     torch.cuda.synchronize()
     wall0 = time.perf_counter()
     for k in range(args.steps):
         ev[k][0].record(stream)
-        if kind == "grouped":
+        if two_pass:
             state.gather(plan)
             ev[k][1].record(stream)
             state.spread(plan)
@@ -249,7 +259,7 @@ def main(argv=None):
     wall = time.perf_counter() - wall0
     log(f"timed {args.steps} orders: {wall:.3f} s")
     step_s = float(np.mean([e[0].elapsed_time(e[-1]) for e in ev])) / 1e3
-    if kind == "grouped":
+    if two_pass:
         gather_s = float(np.mean([e[0].elapsed_time(e[1]) for e in ev])) / 1e3
         spread_s = float(np.mean([e[1].elapsed_time(e[2]) for e in ev])) / 1e3
     state.check()
@@ -271,11 +281,12 @@ def main(argv=None):
     # of the order's histogram and a read + write of the running total
     order_bytes = 4 * U + Vd * (4 * D * B + 8) + 12 * n_state
     kernels = {}
-    if kind == "grouped":
+    if two_pass:
         G = plan.n_groups
-        # gather: gathered bins, per-pair row/delay + per-band weight, one
-        # write of the per-group sums
-        gather_bytes = 4 * U + Vd * (8 + 8 * B) + 4 * G * B * T
+        # gather: gathered bins, per-pair row/delay + per-band weight(s),
+        # one write of the per-group sums
+        n_w = 2 if kind == "ds" else 1
+        gather_bytes = 4 * U + Vd * (8 + 8 * B * n_w) + 4 * G * B * T
         # spread: read the group sums, write cur, read+write hi and lo
         spread_bytes = 4 * G * B * T + 20 * n_state
         kernels = {
@@ -310,7 +321,7 @@ def main(argv=None):
             "patches": P, "directions": D, "bands": B, "bins": T,
             "directed_pairs": Vd, "updates_per_order": U,
             "plan": kind,
-            "groups": plan.n_groups if kind == "grouped" else None,
+            "groups": plan.n_groups if two_pass else None,
         },
         "roofline": {
             "bound": "hbm", "achieved": achieved_gbs, "peak": HBM_PEAK_GBS,

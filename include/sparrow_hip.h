@@ -274,6 +274,57 @@ int spr_exchange_step_grouped(int64_t P, int64_t D, int64_t B, int64_t T,
                               float *gsum, const float *prev, float *cur,
                               float *tot_hi, float *tot_lo, void *stream);
 
+/* Diffuse + specular plan.  create_from_scattering (brdf.py:8-130) makes
+ * every BRDF row a constant s/pi (1-alpha) on all outgoing directions plus
+ * one specular entry, so F~[i,j,:,b] = w_a * 1 + w_c * e_m with one
+ * specular direction m per row: each entry then costs two multiply-adds
+ * instead of D, whatever the number of distinct BRDFs.
+ * Analysis: for the NK = NBRDF*S rows of brdf[NBRDF,S,D,B] (NULL: D = 1,
+ * no BRDF), a[NK,B] and c[NK,B] (float64) and spec_dir[NK] (int32, D when
+ * the row has no specular part); *ok (int32, device) is 1 if every row has
+ * this form, else 0 (then use the grouped or generic plan). */
+int spr_brdf_diffuse_specular(int64_t NK, int64_t D, int64_t B,
+                              const double *brdf, double *a, double *c,
+                              int32_t *spec_dir, int32_t *ok, void *stream);
+
+/* Pass 1 of the plan: like spr_plan_grouped_sort with the entry key
+ * spec_dir[bi(w_i)*S + s_ij]; entries without a specular part (key D)
+ * form no group, and each non-empty row gets one more group, its diffuse
+ * group, last in the row. */
+int spr_plan_ds_sort(int64_t P, int64_t D, int64_t max_row,
+                     const int64_t *row_ptr, int32_t *src,
+                     const double *centers, const int64_t *wall_ids,
+                     int64_t S, const int64_t *brdf_index,
+                     const double *in_dirs, const int32_t *spec_dir,
+                     int32_t *key, int64_t *grp_ptr, void *stream);
+
+/* Pass 2: as spr_plan_grouped_fill, with per-band weights scal[B,Vd] =
+ * F~_ij exp(-air_b |u|) c[k,b] (specular) and dscal[B,Vd] = ... a[k,b]
+ * (diffuse), group keys m (specular direction) or D+1 (the diffuse group,
+ * grp_start = row end), and rtab[D+2,B,DP] (unit vectors, a zero row and a
+ * row of ones), to be used with spr_exchange_gather_ds and
+ * spr_exchange_spread_grouped. */
+int spr_plan_ds_fill(int64_t P, int64_t D, int64_t B, double c, double dt,
+                     int64_t Vd, const int64_t *row_ptr, const int32_t *src,
+                     const int32_t *key, const int64_t *grp_ptr,
+                     const double *centers, const double *areas,
+                     const double *ff, const int64_t *p2o, const double *air,
+                     const int64_t *wall_ids, int64_t S,
+                     const int64_t *brdf_index, const double *in_dirs,
+                     const double *a, const double *cs, int64_t *grp_start,
+                     int32_t *grp_key, int32_t *rowid, int32_t *lag,
+                     double *scal, double *dscal, double *rtab,
+                     void *stream);
+
+/* Gather pass of the diffuse + specular plan: gsum rows of the specular
+ * groups as spr_exchange_gather_grouped with scal, and the row's diffuse
+ * group = sum over all its entries with dscal. */
+int spr_exchange_gather_ds(int64_t P, int64_t B, int64_t T, int64_t Vd,
+                           const int64_t *row_ptr, const int64_t *grp_ptr,
+                           const int32_t *rowid, const int32_t *lag,
+                           const double *scal, const double *dscal,
+                           const float *prev, float *gsum, void *stream);
+
 /* etc[P,D,B,T] (float32) = tot_hi + tot_lo (may alias tot_hi). */
 int spr_exchange_finalize(int64_t n, const float *tot_hi,
                           const float *tot_lo, float *etc, void *stream);

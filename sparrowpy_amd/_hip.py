@@ -88,6 +88,16 @@ SYMBOLS = {
     "spr_exchange_step_grouped": (_int, [_i64, _i64, _i64, _i64, _i64, _vp,
                                          _vp, _vp, _vp, _vp, _vp, _vp, _vp,
                                          _vp, _vp, _vp, _vp, _vp]),
+    "spr_brdf_diffuse_specular": (_int, [_i64, _i64, _i64, _vp, _vp, _vp,
+                                         _vp, _vp, _vp]),
+    "spr_plan_ds_sort": (_int, [_i64, _i64, _i64, _vp, _vp, _vp, _vp, _i64,
+                                _vp, _vp, _vp, _vp, _vp, _vp]),
+    "spr_plan_ds_fill": (_int, [_i64, _i64, _i64, _dbl, _dbl, _i64, _vp, _vp,
+                                _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i64,
+                                _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
+                                _vp, _vp, _vp]),
+    "spr_exchange_gather_ds": (_int, [_i64, _i64, _i64, _i64, _vp, _vp, _vp,
+                                      _vp, _vp, _vp, _vp, _vp, _vp]),
     "spr_energy_exchange_workspace": (_i64, [_i64, _i64, _i64, _i64, _i64]),
     "spr_energy_exchange": (_int, [_i64, _i64, _i64, _i64, _vp, _vp, _vp,
                                    _vp, _vp, _dbl, _dbl, _i64, _i64, _vp,

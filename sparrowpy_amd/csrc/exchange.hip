@@ -460,6 +460,89 @@ __global__ __launch_bounds__(BS) void group_gather_kernel(
   }
 }
 
+// Gather pass of the diffuse + specular plan: as group_gather_kernel, with
+// a second weight per entry (dscal) summed over the whole row into the
+// row's diffuse group (its last group slot).  Written as a runtime loop over
+// pairs of C-entry chunks with scalar metadata loads: with two weights the
+// fully unrolled 64-entry blocks of group_gather_kernel exhaust the
+// register file.
+template <int R, int BS, int C>
+__global__ __launch_bounds__(BS) void ds_gather_kernel(
+    int B, int T, int n_tblk, int64_t nblocks, int64_t Vd,
+    const int64_t *__restrict__ row_ptr, const int64_t *__restrict__ grp_ptr,
+    const int32_t *__restrict__ rowid, const int32_t *__restrict__ lag,
+    const double *__restrict__ scal, const double *__restrict__ dscal,
+    const float *__restrict__ prev, float *__restrict__ gsum) {
+  int64_t j;
+  int b, tb;
+  dest_major_tile(xcd_tile(blockIdx.x, nblocks), nblocks / (B * n_tblk),
+                  n_tblk, j, b, tb);
+  const int t0 = tb * (BS * R);
+  const int t_end = min(T, t0 + BS * R);
+  const int64_t e_begin = row_ptr[j], e_end = row_ptr[j + 1];
+  if (e_end <= e_begin) return;
+  const double *w = scal + (int64_t)b * Vd;
+  const double *w2 = dscal + (int64_t)b * Vd;
+  const uint32_t row_bytes = (uint32_t)T * 4u;
+  const int tl = t0 + (int)threadIdx.x;
+  int64_t g = grp_ptr[j];
+
+  auto fetch = [&](int64_t e0, float (&v)[C][R]) {
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+      const bool in_row = e0 + c < e_end;
+      const int64_t ee = in_row ? e0 + c : e_end - 1;
+      const int n = min(lag[ee] & kLagMask, T);
+      const float *src = prev + ((int64_t)rowid[ee] * B + b) * T;
+      const __amdgpu_buffer_rsrc_t rs =
+          make_rsrc(src, in_row && n < t_end ? row_bytes : 0u);
+#pragma unroll
+      for (int r = 0; r < R; ++r)
+        v[c][r] = buffer_load(rs, (uint32_t)(tl + BS * r - n) * 4u);
+    }
+  };
+  double S[R], Sd[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) S[r] = Sd[r] = 0.0;
+  auto consume = [&](int64_t e0, const float (&v)[C][R]) {
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+      const int64_t e = e0 + c;
+      if (e >= e_end) break;
+      const double wk = w[e], dk = w2[e];
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const double x = (double)v[c][r];
+        S[r] = fma(wk, x, S[r]);
+        Sd[r] = fma(dk, x, Sd[r]);
+      }
+      if (lag[e] & kLastInGroup) {
+        const __amdgpu_buffer_rsrc_t os =
+            make_rsrc(gsum + (g * B + b) * (int64_t)T, row_bytes);
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          buffer_store((float)S[r], os, (uint32_t)(tl + BS * r) * 4u);
+          S[r] = 0.0;
+        }
+        g += 1;
+      }
+    }
+  };
+  float va[C][R], vb[C][R];
+  fetch(e_begin, va);
+  for (int64_t e = e_begin; e < e_end; e += 2 * C) {
+    fetch(e + C, vb);
+    consume(e, va);
+    fetch(e + 2 * C, va);
+    consume(e + C, vb);
+  }
+  const __amdgpu_buffer_rsrc_t os = make_rsrc(
+      gsum + ((grp_ptr[j + 1] - 1) * B + b) * (int64_t)T, row_bytes);
+#pragma unroll
+  for (int r = 0; r < R; ++r)
+    buffer_store((float)Sd[r], os, (uint32_t)(tl + BS * r) * 4u);
+}
+
 // Grouped plan, pass 2: spread the group sums over the D directions with
 // their BRDF rows and update the order histogram and the running total:
 //   cur[j, :, b, t] = sum_{g of j} rtab[key_g, b, :] * gsum[g, b, t].
@@ -621,17 +704,23 @@ constexpr int kGatherC = 8;
 int launch_gather(int64_t P, int64_t B, int64_t T, int64_t Vd,
                   const int64_t *row_ptr, const int64_t *grp_ptr,
                   const int32_t *rowid, const int32_t *lag,
-                  const double *scal, const float *prev, float *gsum,
-                  hipStream_t s) {
+                  const double *scal, const double *dscal, const float *prev,
+                  float *gsum, hipStream_t s) {
   constexpr int BS = 256;
   const int64_t nt = ceil_div(T, (int64_t)BS * kGatherR);
   const int64_t nb = P * B * nt;
   SPR_REQUIRE(nb < (int64_t)0x7fffffff, "exchange grid too large");
   SPR_REQUIRE(T < ((int64_t)1 << 29), "histogram too long (T >= 2^29)");
-  hipLaunchKernelGGL((group_gather_kernel<kGatherR, BS, kGatherC>),
-                     dim3((unsigned)nb), dim3(BS), 0, s, (int)B, (int)T,
-                     (int)nt, nb, Vd, row_ptr, grp_ptr, rowid, lag, scal,
-                     prev, gsum);
+  if (dscal)
+    hipLaunchKernelGGL((ds_gather_kernel<kGatherR, BS, kGatherC>),
+                       dim3((unsigned)nb), dim3(BS), 0, s, (int)B, (int)T,
+                       (int)nt, nb, Vd, row_ptr, grp_ptr, rowid, lag, scal,
+                       dscal, prev, gsum);
+  else
+    hipLaunchKernelGGL((group_gather_kernel<kGatherR, BS, kGatherC>),
+                       dim3((unsigned)nb), dim3(BS), 0, s, (int)B, (int)T,
+                       (int)nt, nb, Vd, row_ptr, grp_ptr, rowid, lag, scal,
+                       prev, gsum);
   SPR_LAUNCH_CHECK();
   return SPR_OK;
 }
@@ -658,35 +747,46 @@ int launch_spread(int64_t P, int64_t D, int64_t B, int64_t T,
 // ---------------------------------------------------------------------
 constexpr int kSortMax = 8192;  // longest destination row the LDS sort takes
 
-// Key of entry e (source i -> destination j): the BRDF row its coefficient
-// vector comes from, brdf_index[w_i] * S + s_ij (0 without a BRDF).
+// BRDF row of entry e (source i -> destination j): brdf_index[w_i] * S +
+// s_ij (0 without a BRDF).
+__device__ __forceinline__ int64_t entry_brdf_row(
+    int64_t i, int64_t j, const double *__restrict__ centers,
+    const int64_t *__restrict__ wall_ids, int64_t S,
+    const int64_t *__restrict__ brdf_index,
+    const double *__restrict__ in_dirs) {
+  if (!in_dirs) return 0;
+  d3 u = load3(centers + 3 * i) - load3(centers + 3 * j);
+  u = div(u, norm(u));
+  const int64_t w = wall_ids[i];
+  return brdf_index[w] * S + nearest_direction(in_dirs + w * S * 3, S, u);
+}
+
+// Key of entry e: its BRDF row, or with spec_dir (diffuse + specular plan)
+// the specular direction of that row.
 __global__ __launch_bounds__(256) void group_key_kernel(
     int64_t P, const int64_t *__restrict__ row_ptr,
     const int32_t *__restrict__ src, const double *__restrict__ centers,
     const int64_t *__restrict__ wall_ids, int64_t S,
     const int64_t *__restrict__ brdf_index,
-    const double *__restrict__ in_dirs, int32_t *__restrict__ key) {
+    const double *__restrict__ in_dirs,
+    const int32_t *__restrict__ spec_dir, int32_t *__restrict__ key) {
   const int64_t j = blockIdx.x;
   for (int64_t e = row_ptr[j] + threadIdx.x; e < row_ptr[j + 1];
        e += blockDim.x) {
-    if (!in_dirs) {
-      key[e] = 0;
-      continue;
-    }
-    const int64_t i = src[e];
-    d3 u = load3(centers + 3 * i) - load3(centers + 3 * j);
-    u = div(u, norm(u));
-    const int64_t w = wall_ids[i];
-    const int64_t sidx = nearest_direction(in_dirs + w * S * 3, S, u);
-    key[e] = (int32_t)(brdf_index[w] * S + sidx);
+    const int64_t k = entry_brdf_row(src[e], j, centers, wall_ids, S,
+                                     brdf_index, in_dirs);
+    key[e] = spec_dir ? spec_dir[k] : (int32_t)k;
   }
 }
 
 // Sort each destination row by (key, position) with a bitonic sort in LDS
 // (stable: ties keep ascending source order) and count its groups.
+// Entries whose key is none_key form no group; a non-empty row gets
+// `extra` more groups (the diffuse group of the diffuse + specular plan).
 __global__ __launch_bounds__(1024) void group_sort_kernel(
     int64_t P, const int64_t *__restrict__ row_ptr, int32_t *__restrict__ src,
-    int32_t *__restrict__ key, int64_t *__restrict__ grp_count, int npad) {
+    int32_t *__restrict__ key, int64_t *__restrict__ grp_count, int npad,
+    int32_t none_key, int extra) {
   // dynamic LDS only (8-byte aligned base): keys, source ids, group count
   extern __shared__ uint64_t smem64[];
   uint64_t *k = smem64;
@@ -726,18 +826,23 @@ __global__ __launch_bounds__(1024) void group_sort_kernel(
     const uint32_t kq = (uint32_t)(k[q] >> 32);
     src[e0 + q] = sv[(uint32_t)(k[q] & 0xffffffffu)];
     key[e0 + q] = (int32_t)kq;
-    if (q == 0 || (uint32_t)(k[q - 1] >> 32) != kq) local += 1;
+    if ((int32_t)kq != none_key &&
+        (q == 0 || (uint32_t)(k[q - 1] >> 32) != kq))
+      local += 1;
   }
   atomicAdd(&groups, local);
   __syncthreads();
-  if (threadIdx.x == 0) grp_count[j] = groups;
+  if (threadIdx.x == 0) grp_count[j] = groups + (n > 0 ? extra : 0);
 }
 
 // Group starts and keys, one wavefront per destination row.
+// With diffuse_key >= 0 a non-empty row's last group is its diffuse group
+// (no entries of its own: start = row end, key = diffuse_key).
 __global__ __launch_bounds__(256) void group_fill_kernel(
     int64_t P, int64_t Vd, const int64_t *__restrict__ row_ptr,
     const int32_t *__restrict__ key, const int64_t *__restrict__ grp_ptr,
-    int64_t *__restrict__ grp_start, int32_t *__restrict__ grp_key) {
+    int64_t *__restrict__ grp_start, int32_t *__restrict__ grp_key,
+    int32_t none_key, int32_t diffuse_key) {
   const int lane = threadIdx.x & 63;
   const int64_t j = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
   if (j >= P) return;
@@ -745,7 +850,8 @@ __global__ __launch_bounds__(256) void group_fill_kernel(
   int64_t pos = grp_ptr[j];
   for (int64_t base = e0; base < e1; base += 64) {
     const int64_t e = base + lane;
-    const bool f = e < e1 && (e == e0 || key[e] != key[e - 1]);
+    const bool f = e < e1 && key[e] != none_key &&
+                   (e == e0 || key[e] != key[e - 1]);
     const uint64_t m = __ballot(f);
     if (f) {
       const uint64_t lt = lane ? (m & ((1ull << lane) - 1ull)) : 0ull;
@@ -754,6 +860,10 @@ __global__ __launch_bounds__(256) void group_fill_kernel(
       grp_key[g] = key[e];
     }
     pos += __popcll(m);
+  }
+  if (diffuse_key >= 0 && e1 > e0 && lane == 0) {
+    grp_start[grp_ptr[j + 1] - 1] = e1;
+    grp_key[grp_ptr[j + 1] - 1] = diffuse_key;
   }
   if (j == P - 1 && lane == 0) grp_start[grp_ptr[P]] = Vd;
 }
@@ -795,6 +905,98 @@ __global__ void rtab_kernel(int64_t NK, int64_t D, int64_t B, int64_t DP,
   const int64_t d = q % DP, b = (q / DP) % B, k = q / (DP * B);
   double v = 0.0;
   if (d < D) v = brdf ? brdf[(k * D + d) * B + b] : 1.0;
+  rtab[q] = v;
+}
+
+// Diffuse + specular decomposition of BRDF rows: row k (brdf[k, :, b]) is
+// a_kb on every direction except at most one, spec_dir[k] = m, where it is
+// a_kb + c_kb (the form create_from_scattering produces: s/pi everywhere
+// plus the specular lobe in one direction, times 1 - alpha).  m is the
+// first maximum; rows whose specular part vanishes in every band get
+// m = D (no specular group).  *ok is cleared if any row has another form.
+__global__ void ds_table_kernel(int64_t NK, int64_t D, int64_t B,
+                                const double *__restrict__ brdf,
+                                double *__restrict__ a, double *__restrict__ c,
+                                int32_t *__restrict__ spec_dir,
+                                int32_t *__restrict__ ok) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= NK) return;
+  if (!brdf) {
+    for (int64_t b = 0; b < B; ++b) {
+      a[k * B + b] = 1.0;
+      c[k * B + b] = 0.0;
+    }
+    spec_dir[k] = (int32_t)D;
+    return;
+  }
+  const double *row = brdf + k * D * B;
+  int64_t m = D;
+  bool good = true;
+  for (int64_t b = 0; b < B; ++b) {
+    int64_t mb = 0;
+    for (int64_t d = 1; d < D; ++d)
+      if (row[d * B + b] > row[mb * B + b]) mb = d;
+    const double base = row[(mb == 0 && D > 1 ? 1 : 0) * B + b];
+    for (int64_t d = 0; d < D; ++d)
+      if (d != mb && row[d * B + b] != base) good = false;
+    const double spec = D > 1 ? row[mb * B + b] - base : 0.0;
+    a[k * B + b] = D > 1 ? base : row[b];
+    c[k * B + b] = spec;
+    if (spec != 0.0) {
+      if (m != D && m != mb) good = false;
+      m = mb;
+    }
+  }
+  spec_dir[k] = (int32_t)m;
+  if (!good) atomicMin(ok, 0);
+}
+
+// Entries of the diffuse + specular plan: delay (flagged at the end of a
+// specular group), source row, and per band the specular weight
+// scal = F~_ij exp(-air_b |u|) c_kb and the diffuse weight ... a_kb.
+__global__ __launch_bounds__(256) void ds_entry_kernel(
+    int64_t P, int64_t D, int64_t B, double c, double dt, int64_t Vd,
+    const int64_t *__restrict__ row_ptr, const int32_t *__restrict__ src,
+    const int32_t *__restrict__ key, const double *__restrict__ centers,
+    const double *__restrict__ areas, const double *__restrict__ ff,
+    const int64_t *__restrict__ p2o, const double *__restrict__ air,
+    const int64_t *__restrict__ wall_ids, int64_t S,
+    const int64_t *__restrict__ brdf_index,
+    const double *__restrict__ in_dirs, const double *__restrict__ ta,
+    const double *__restrict__ tc, int32_t *__restrict__ rowid,
+    int32_t *__restrict__ lag, double *__restrict__ scal,
+    double *__restrict__ dscal) {
+  const int64_t j = blockIdx.x;
+  const int64_t e1 = row_ptr[j + 1];
+  for (int64_t e = row_ptr[j] + threadIdx.x; e < e1; e += blockDim.x) {
+    const int64_t i = src[e];
+    const double dij = norm(load3(centers + 3 * i) - load3(centers + 3 * j));
+    const bool last = key[e] != (int32_t)D &&
+                      (e + 1 == e1 || key[e + 1] != key[e]);
+    lag[e] = clamp_lag(delay_floor(dij, c, dt)) | (last ? kLastInGroup : 0);
+    rowid[e] = (int32_t)(i * D + checked_dir(p2o[i * P + j], D));
+    const int64_t k = entry_brdf_row(i, j, centers, wall_ids, S, brdf_index,
+                                     in_dirs);
+    PairFactor pf = pair_factor(i, j, P, centers, areas, ff, nullptr, 1, 1, B,
+                                nullptr, nullptr, nullptr);
+    for (int64_t b = 0; b < B; ++b) {
+      double v = pf.f;
+      if (air) v = v * exp(-air[b] * pf.distance);
+      scal[b * Vd + e] = v * tc[k * B + b];
+      dscal[b * Vd + e] = v * ta[k * B + b];
+    }
+  }
+}
+
+// rtab of the diffuse + specular plan: row m < D the unit vector e_m,
+// row D zeros, row D + 1 ones (padding zero).
+__global__ void ds_rtab_kernel(int64_t D, int64_t B, int64_t DP,
+                               double *__restrict__ rtab) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= (D + 2) * B * DP) return;
+  const int64_t d = q % DP, k = q / (DP * B);
+  double v = 0.0;
+  if (d < D) v = (k == D + 1) ? 1.0 : (k == d ? 1.0 : 0.0);
   rtab[q] = v;
 }
 
@@ -948,6 +1150,33 @@ extern "C" int spr_exchange_finalize(int64_t n, const float *tot_hi,
   return SPR_OK;
 }
 
+namespace spr {
+namespace {
+// Keys, per-row stable sort by key and exclusive scan of the group counts.
+int sort_rows(int64_t P, int64_t max_row, const int64_t *row_ptr,
+              int32_t *src, const double *centers, const int64_t *wall_ids,
+              int64_t S, const int64_t *brdf_index, const double *in_dirs,
+              const int32_t *spec_dir, int32_t none_key, int extra,
+              int32_t *key, int64_t *grp_ptr, hipStream_t s) {
+  hipLaunchKernelGGL(group_key_kernel, dim3((unsigned)P), dim3(256), 0, s, P,
+                     row_ptr, src, centers, wall_ids, S, brdf_index, in_dirs,
+                     spec_dir, key);
+  SPR_LAUNCH_CHECK();
+  int npad = 1;
+  while (npad < max_row) npad <<= 1;
+  const size_t lds =
+      (size_t)npad * (sizeof(uint64_t) + sizeof(int32_t)) + sizeof(int32_t);
+  SPR_HIP(hipFuncSetAttribute((const void *)group_sort_kernel,
+                              hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)lds));
+  hipLaunchKernelGGL(group_sort_kernel, dim3((unsigned)P), dim3(1024), lds, s,
+                     P, row_ptr, src, key, grp_ptr, npad, none_key, extra);
+  SPR_LAUNCH_CHECK();
+  return scan_exclusive(grp_ptr, P, s);
+}
+}  // namespace
+}  // namespace spr
+
 extern "C" int spr_plan_grouped_sort(
     int64_t P, int64_t max_row, const int64_t *row_ptr, int32_t *src,
     const double *centers, const int64_t *wall_ids, int64_t S,
@@ -964,21 +1193,8 @@ extern "C" int spr_plan_grouped_sort(
                 "the grouped plan supports %d",
                 (long long)max_row, kSortMax);
   hipStream_t s = as_stream(stream);
-  hipLaunchKernelGGL(group_key_kernel, dim3((unsigned)P), dim3(256), 0, s, P,
-                     row_ptr, src, centers, wall_ids, S, brdf_index, in_dirs,
-                     key);
-  SPR_LAUNCH_CHECK();
-  int npad = 1;
-  while (npad < max_row) npad <<= 1;
-  const size_t lds =
-      (size_t)npad * (sizeof(uint64_t) + sizeof(int32_t)) + sizeof(int32_t);
-  SPR_HIP(hipFuncSetAttribute((const void *)group_sort_kernel,
-                              hipFuncAttributeMaxDynamicSharedMemorySize,
-                              (int)lds));
-  hipLaunchKernelGGL(group_sort_kernel, dim3((unsigned)P), dim3(1024), lds, s,
-                     P, row_ptr, src, key, grp_ptr, npad);
-  SPR_LAUNCH_CHECK();
-  return scan_exclusive(grp_ptr, P, s);
+  return sort_rows(P, max_row, row_ptr, src, centers, wall_ids, S,
+                   brdf_index, in_dirs, nullptr, -1, 0, key, grp_ptr, s);
 }
 
 extern "C" int spr_plan_grouped_fill(
@@ -1001,7 +1217,7 @@ extern "C" int spr_plan_grouped_fill(
   hipStream_t s = as_stream(stream);
   hipLaunchKernelGGL(group_fill_kernel, dim3((unsigned)ceil_div(P, 4)),
                      dim3(256), 0, s, P, Vd, row_ptr, key, grp_ptr, grp_start,
-                     grp_key);
+                     grp_key, (int32_t)-1, (int32_t)-1);
   SPR_LAUNCH_CHECK();
   hipLaunchKernelGGL(group_entry_kernel, dim3((unsigned)P), dim3(256), 0, s,
                      P, D, B, c, dt, Vd, row_ptr, src, key, centers, areas, ff,
@@ -1014,6 +1230,100 @@ extern "C" int spr_plan_grouped_fill(
   return SPR_OK;
 }
 
+extern "C" int spr_brdf_diffuse_specular(int64_t NK, int64_t D, int64_t B,
+                                         const double *brdf, double *a,
+                                         double *c, int32_t *spec_dir,
+                                         int32_t *ok, void *stream) {
+  SPR_REQUIRE(NK > 0 && D > 0 && B > 0, "spr_brdf_diffuse_specular: bad sizes");
+  SPR_REQUIRE(a && c && spec_dir && ok,
+              "spr_brdf_diffuse_specular: null pointer");
+  SPR_REQUIRE(brdf || D == 1,
+              "spr_brdf_diffuse_specular: D must be 1 without a BRDF");
+  hipStream_t s = as_stream(stream);
+  int rc = fill<int32_t>(ok, 1, 1, s);
+  if (rc) return rc;
+  hipLaunchKernelGGL(ds_table_kernel, dim3((unsigned)ceil_div(NK, 256)),
+                     dim3(256), 0, s, NK, D, B, brdf, a, c, spec_dir, ok);
+  SPR_LAUNCH_CHECK();
+  return SPR_OK;
+}
+
+extern "C" int spr_plan_ds_sort(int64_t P, int64_t D, int64_t max_row,
+                                const int64_t *row_ptr, int32_t *src,
+                                const double *centers,
+                                const int64_t *wall_ids, int64_t S,
+                                const int64_t *brdf_index,
+                                const double *in_dirs,
+                                const int32_t *spec_dir, int32_t *key,
+                                int64_t *grp_ptr, void *stream) {
+  SPR_REQUIRE(P > 0 && D > 0 && max_row >= 0, "spr_plan_ds_sort: bad sizes");
+  SPR_REQUIRE(row_ptr && src && centers && key && grp_ptr && spec_dir,
+              "spr_plan_ds_sort: null pointer");
+  SPR_REQUIRE(!in_dirs || (wall_ids && brdf_index && S > 0),
+              "spr_plan_ds_sort: in_dirs needs wall_ids, brdf_index");
+  if (max_row > kSortMax)
+    return fail(SPR_ERR_UNSUPPORTED,
+                "spr_plan_ds_sort: a destination sees %lld patches, "
+                "the plan supports %d",
+                (long long)max_row, kSortMax);
+  return sort_rows(P, max_row, row_ptr, src, centers, wall_ids, S,
+                   brdf_index, in_dirs, spec_dir, (int32_t)D, 1, key, grp_ptr,
+                   as_stream(stream));
+}
+
+extern "C" int spr_plan_ds_fill(
+    int64_t P, int64_t D, int64_t B, double c, double dt, int64_t Vd,
+    const int64_t *row_ptr, const int32_t *src, const int32_t *key,
+    const int64_t *grp_ptr, const double *centers, const double *areas,
+    const double *ff, const int64_t *p2o, const double *air,
+    const int64_t *wall_ids, int64_t S, const int64_t *brdf_index,
+    const double *in_dirs, const double *ta, const double *tc,
+    int64_t *grp_start, int32_t *grp_key, int32_t *rowid, int32_t *lag,
+    double *scal, double *dscal, double *rtab, void *stream) {
+  const int64_t DP = spr_padded_directions(D);
+  SPR_REQUIRE(DP > 0, "spr_plan_ds_fill: D=%lld unsupported (max 64)",
+              (long long)D);
+  SPR_REQUIRE(P > 0 && B > 0 && c > 0 && dt > 0 && Vd >= 0,
+              "spr_plan_ds_fill: bad sizes");
+  SPR_REQUIRE(row_ptr && src && key && grp_ptr && centers && areas && ff &&
+                  p2o && ta && tc && grp_start && grp_key && rowid && lag &&
+                  scal && dscal && rtab,
+              "spr_plan_ds_fill: null pointer");
+  SPR_REQUIRE(!in_dirs || (wall_ids && brdf_index && S > 0),
+              "spr_plan_ds_fill: in_dirs needs wall_ids, brdf_index");
+  hipStream_t s = as_stream(stream);
+  hipLaunchKernelGGL(group_fill_kernel, dim3((unsigned)ceil_div(P, 4)),
+                     dim3(256), 0, s, P, Vd, row_ptr, key, grp_ptr, grp_start,
+                     grp_key, (int32_t)D, (int32_t)(D + 1));
+  SPR_LAUNCH_CHECK();
+  if (Vd > 0) {
+    hipLaunchKernelGGL(ds_entry_kernel, dim3((unsigned)P), dim3(256), 0, s, P,
+                       D, B, c, dt, Vd, row_ptr, src, key, centers, areas, ff,
+                       p2o, air, wall_ids, S, brdf_index, in_dirs, ta, tc,
+                       rowid, lag, scal, dscal);
+    SPR_LAUNCH_CHECK();
+  }
+  const int64_t nr = (D + 2) * B * DP;
+  hipLaunchKernelGGL(ds_rtab_kernel, dim3((unsigned)ceil_div(nr, 256)),
+                     dim3(256), 0, s, D, B, DP, rtab);
+  SPR_LAUNCH_CHECK();
+  return SPR_OK;
+}
+
+extern "C" int spr_exchange_gather_ds(
+    int64_t P, int64_t B, int64_t T, int64_t Vd, const int64_t *row_ptr,
+    const int64_t *grp_ptr, const int32_t *rowid, const int32_t *lag,
+    const double *scal, const double *dscal, const float *prev, float *gsum,
+    void *stream) {
+  SPR_REQUIRE(P > 0 && B > 0 && T > 0 && Vd >= 0,
+              "spr_exchange_gather_ds: bad sizes");
+  SPR_REQUIRE(row_ptr && grp_ptr && rowid && lag && scal && dscal && prev &&
+                  gsum,
+              "spr_exchange_gather_ds: null pointer");
+  return launch_gather(P, B, T, Vd, row_ptr, grp_ptr, rowid, lag, scal,
+                       dscal, prev, gsum, as_stream(stream));
+}
+
 extern "C" int spr_exchange_gather_grouped(
     int64_t P, int64_t B, int64_t T, int64_t Vd, const int64_t *row_ptr,
     const int64_t *grp_ptr, const int32_t *rowid, const int32_t *lag,
@@ -1022,8 +1332,8 @@ extern "C" int spr_exchange_gather_grouped(
               "spr_exchange_gather_grouped: bad sizes");
   SPR_REQUIRE(row_ptr && grp_ptr && rowid && lag && scal && prev && gsum,
               "spr_exchange_gather_grouped: null pointer");
-  return launch_gather(P, B, T, Vd, row_ptr, grp_ptr, rowid, lag, scal, prev,
-                       gsum, as_stream(stream));
+  return launch_gather(P, B, T, Vd, row_ptr, grp_ptr, rowid, lag, scal,
+                       nullptr, prev, gsum, as_stream(stream));
 }
 
 extern "C" int spr_exchange_spread_grouped(

@@ -375,10 +375,15 @@ class GroupedPlan:
     lag: torch.Tensor
     scal: torch.Tensor
     rtab: torch.Tensor
+    dscal: torch.Tensor = None  # diffuse weights (diffuse + specular plan)
 
     @property
     def Vd(self):
         return int(self.rowid.numel())
+
+    @property
+    def diffuse_specular(self):
+        return self.dscal is not None
 
     @property
     def n_groups(self):
@@ -437,6 +442,80 @@ class GroupedPlan:
                    rowid[:Vd], lag[:Vd], scal[:, :Vd].contiguous(), rtab)
 
 
+def diffuse_specular_tables(scattering, n_directions, n_bins, n_rows):
+    """Split BRDF rows into a diffuse constant and one specular entry
+    (``spr_brdf_diffuse_specular``).  Returns (a [NK,B], c [NK,B],
+    spec_dir [NK]) device tensors, or None if a row has another form."""
+    D, B = int(n_directions), int(n_bins)
+    brdf = None if scattering is None else _f64(scattering)
+    NK = int(n_rows)
+    dev = device()
+    a = torch.empty((NK, B), dtype=torch.float64, device=dev)
+    c = torch.empty((NK, B), dtype=torch.float64, device=dev)
+    m = torch.empty(NK, dtype=torch.int32, device=dev)
+    ok = torch.empty(1, dtype=torch.int32, device=dev)
+    call("spr_brdf_diffuse_specular", NK, D, B, ptr(brdf), ptr(a), ptr(c),
+         ptr(m), ptr(ok), stream_ptr())
+    return (a, c, m) if int(ok.item()) else None
+
+
+def diffuse_specular_plan(vis, patches_center, patches_area, form_factors,
+                          p2o, air_attenuation, patch_to_wall_ids, sources,
+                          scattering, scattering_index, n_directions, n_bins,
+                          speed_of_sound, time_resolution):
+    """GroupedPlan of the diffuse + specular form (see sparrow_hip.h), or
+    None if some BRDF row is not a constant plus one specular entry."""
+    c = _f64(patches_center)
+    dev = c.device
+    P = c.shape[0]
+    D, B = int(n_directions), int(n_bins)
+    DP = int(_hip.lib().spr_padded_directions(D))
+    if DP == 0:
+        raise ValueError(f"{D} directions unsupported (max 64)")
+    if scattering is None:
+        S, NK, bidx, sd = 1, 1, None, None
+    else:
+        sd = _f64(sources)
+        S = sd.shape[1]
+        bidx = _i64(scattering_index)
+        NK = int(np.shape(scattering)[0]) * S
+    tables = diffuse_specular_tables(scattering, D, B, NK)
+    if tables is None:
+        return None
+    ta, tc, spec_dir = tables
+    row_ptr, src = ExchangePlan.rows(vis)
+    Vd = src.numel()
+    max_row = int((row_ptr[1:] - row_ptr[:-1]).max().item()) if P else 0
+    if max_row > GroupedPlan.MAX_ROW:
+        raise ValueError("destination rows too long for the grouped plan")
+    src = src.clone() if Vd else torch.empty(1, dtype=torch.int32,
+                                             device=dev)
+    w = _i64(patch_to_wall_ids)
+    key = torch.empty(max(Vd, 1), dtype=torch.int32, device=dev)
+    grp_ptr = torch.empty(P + 1, dtype=torch.int64, device=dev)
+    call("spr_plan_ds_sort", P, D, max_row, ptr(row_ptr), ptr(src), ptr(c),
+         ptr(w), S, ptr(bidx), ptr(sd), ptr(spec_dir), ptr(key),
+         ptr(grp_ptr), stream_ptr())
+    G = int(grp_ptr[P].item())
+    grp_start = torch.empty(G + 1, dtype=torch.int64, device=dev)
+    grp_key = torch.empty(max(G, 1), dtype=torch.int32, device=dev)[:G]
+    rowid = torch.empty(max(Vd, 1), dtype=torch.int32, device=dev)
+    lag = torch.empty(max(Vd, 1), dtype=torch.int32, device=dev)
+    scal = torch.empty((B, max(Vd, 1)), dtype=torch.float64, device=dev)
+    dscal = torch.empty((B, max(Vd, 1)), dtype=torch.float64, device=dev)
+    rtab = torch.empty((D + 2, B, DP), dtype=torch.float64, device=dev)
+    call("spr_plan_ds_fill", P, D, B, float(speed_of_sound),
+         float(time_resolution), Vd, ptr(row_ptr), ptr(src), ptr(key),
+         ptr(grp_ptr), ptr(c), ptr(_f64(patches_area)),
+         ptr(_f64(form_factors)), ptr(_i64(p2o)), ptr(_f64(air_attenuation)),
+         ptr(w), S, ptr(bidx), ptr(sd), ptr(ta), ptr(tc), ptr(grp_start),
+         ptr(grp_key), ptr(rowid), ptr(lag), ptr(scal), ptr(dscal),
+         ptr(rtab), stream_ptr())
+    return GroupedPlan(P, D, B, row_ptr, grp_ptr, grp_start, grp_key,
+                       rowid[:Vd], lag[:Vd], scal[:, :Vd].contiguous(), rtab,
+                       dscal[:, :Vd].contiguous())
+
+
 class ExchangeState:
     """Order histograms and the compensated running total of one source.
 
@@ -490,6 +569,12 @@ class ExchangeState:
     def gather(self, plan):
         """Grouped plan, pass 1 of an order (per-group sums)."""
         P, D, B, T = self.shape
+        if plan.diffuse_specular:
+            call("spr_exchange_gather_ds", P, B, T, plan.Vd,
+                 ptr(plan.row_ptr), ptr(plan.grp_ptr), ptr(plan.rowid),
+                 ptr(plan.lag), ptr(plan.scal), ptr(plan.dscal),
+                 ptr(self.prev), ptr(self._gsum(plan)), stream_ptr())
+            return
         call("spr_exchange_gather_grouped", P, B, T, plan.Vd,
              ptr(plan.row_ptr), ptr(plan.grp_ptr), ptr(plan.rowid),
              ptr(plan.lag), ptr(plan.scal), ptr(self.prev),

@@ -475,18 +475,16 @@ class DirectionalRadiosityFast:
     def _exchange_plan(self, speed_of_sound, time_resolution, kind=None):
         """Plan of the exchange (cached per speed of sound and bin width).
 
-        ``kind``: "grouped", "generic" or None (automatic: grouped when the
-        sources of a destination share few BRDF rows).
+        ``kind``: "grouped", "ds" (diffuse + specular BRDF rows), "generic"
+        or None (automatic: "grouped" when the sources of a destination
+        share few BRDF rows, else "ds" when every BRDF row is a constant
+        plus one specular entry, else "generic").
         """
         key = (float(speed_of_sound), float(time_resolution), kind)
         if getattr(self, "_plan", None) is not None and self._plan[0] == key:
             return self._plan[1]
         b = self._bake
         g = self._geometry()
-        args = (b["vis"], g["centers"], g["areas"], b["ff"], b["p2o"],
-                b["air"], g["wall_ids"], b["sources"], b["brdf"],
-                b["brdf_index"], b["D"], b["n_bins"], speed_of_sound,
-                time_resolution)
         plan = None
         if b.get("fft_dense") is not None:
             # a saved F~ is used as given (reference: the attribute read
@@ -500,6 +498,10 @@ class DirectionalRadiosityFast:
                 time_resolution)
             self._plan = (key, plan)
             return plan
+        args = (b["vis"], g["centers"], g["areas"], b["ff"], b["p2o"],
+                b["air"], g["wall_ids"], b["sources"], b["brdf"],
+                b["brdf_index"], b["D"], b["n_bins"], speed_of_sound,
+                time_resolution)
         if kind in (None, "grouped"):
             try:
                 plan = ops.GroupedPlan.from_geometry(*args)
@@ -509,6 +511,14 @@ class DirectionalRadiosityFast:
             if (plan is not None and kind is None and plan.Vd and
                     plan.n_groups > self.GROUPED_MAX_GROUP_FRACTION * plan.Vd):
                 plan = None
+        if plan is None and kind in (None, "ds"):
+            try:
+                plan = ops.diffuse_specular_plan(*args)
+            except ValueError:
+                if kind == "ds":
+                    raise
+            if plan is None and kind == "ds":
+                raise ValueError("BRDF rows are not diffuse + specular")
         if plan is None:
             plan = ops.ExchangePlan.from_geometry(*args)
         self._plan = (key, plan)

@@ -349,3 +349,122 @@ def test_grouped_plan_without_brdf(ops):
                        r.pair_distances(), r.fft, r.p2o, c, dt, K,
                        r.visible_patches)
     assert rel_err_within_db(_np(st.finalize()), ref) <= EXCHANGE_RTOL
+
+
+def _ds_plan(ops, r, c, dt):
+    return ops.diffuse_specular_plan(
+        r.visibility_matrix, r.patches_center, r.patches_area,
+        r.form_factors, r.p2o, r.air, r.patch_to_wall_ids, r.brdf_in,
+        np.array(r.brdf), r.brdf_index, r.brdf_out.shape[1], r.n_bins, c, dt)
+
+
+@pytest.mark.parametrize("per_wall", [True, False])
+def test_ds_plan_contents(ops, per_wall):
+    """Diffuse + specular plan: per row, the entries are the generic row's
+    (same source rows and delays), dscal + scal * e_key reproduces every
+    generic coefficient row, groups are keyed by distinct specular
+    directions, and each non-empty row ends with its diffuse group."""
+    r = _oracle_directional(testing.shoebox_room_stub(3, 2.5, 2), .5,
+                            per_wall=per_wall, seed=3)
+    c, dt = 343., 1 / 2000
+    ds = _ds_plan(ops, r, c, dt)
+    assert ds is not None and ds.diffuse_specular
+    _, gen = _grouped_and_generic(ops, r, c, dt)
+    P, D, B = r.n_patches, r.brdf_out.shape[1], r.n_bins
+    rp = _np(gen.row_ptr)
+    np.testing.assert_array_equal(_np(ds.row_ptr), rp)
+    gptr, gk, gs = _np(ds.grp_ptr), _np(ds.grp_key), _np(ds.grp_start)
+    flag = ops.GroupedPlan.LAST_IN_GROUP
+    lag_ds = _np(ds.lag)
+    a = _np(gen.src).astype(np.int64) * D + _np(gen.dir)
+    b = _np(ds.rowid).astype(np.int64)
+    cg, lg = _np(gen.coef), _np(gen.lag)
+    sc, dsc = _np(ds.scal).T, _np(ds.dscal).T   # [Vd, B]
+    for j in range(P):
+        s = slice(rp[j], rp[j + 1])
+        if rp[j + 1] == rp[j]:
+            assert gptr[j + 1] == gptr[j]
+            continue
+        keys = gk[gptr[j]:gptr[j + 1]]
+        assert keys[-1] == D + 1 and gs[gptr[j + 1] - 1] == rp[j + 1]
+        assert np.all(np.diff(keys[:-1]) > 0) and np.all(keys[:-1] < D)
+        # specular direction of every entry from its group: groups are
+        # consecutive, each ends at a flagged entry; the rest have none
+        ends = np.nonzero(lag_ds[s] & flag)[0]
+        assert len(ends) == gptr[j + 1] - gptr[j] - 1
+        spec = np.full(rp[j + 1] - rp[j], -1)
+        start = 0
+        for q, g in enumerate(range(gptr[j], gptr[j + 1] - 1)):
+            assert gs[g] - rp[j] == start
+            spec[start:ends[q] + 1] = gk[g]
+            start = ends[q] + 1
+        coef = np.repeat(dsc[s][:, :, None], D, axis=2)
+        for q in np.nonzero(spec >= 0)[0]:
+            coef[q, :, spec[q]] += sc[s][q]
+        oa, ob = np.argsort(a[s], kind="stable"), np.argsort(b[s],
+                                                            kind="stable")
+        np.testing.assert_array_equal(a[s][oa], b[s][ob])
+        np.testing.assert_array_equal(lg[s][oa], (lag_ds[s] & (flag - 1))[ob])
+        np.testing.assert_allclose(coef[ob], cg[s][oa][:, :, :D],
+                                   rtol=1e-13, atol=1e-300)
+
+
+@pytest.mark.parametrize("per_wall", [True, False])
+def test_ds_plan_matches_oracle(ops, per_wall):
+    r = _oracle_directional(testing.shoebox_room_stub(3, 2.5, 2), .5,
+                            per_wall=per_wall, seed=4)
+    r.init_source_energy(np.array([1.1, .8, .6]))
+    T, c, dt, K = 400, 343., 1 / 2000, 8
+    plan = _ds_plan(ops, r, c, dt)
+    P, D, B = r.energy_init_source.shape
+    st = ops.ExchangeState(P, D, B, T)
+    st.init(r.energy_init_source, r.distance_patches_to_source, c, dt)
+    for _ in range(K):
+        st.step(plan)
+    out = _np(st.finalize())
+    st.check()
+    ref = orc.exchange(T, r.energy_init_source, r.distance_patches_to_source,
+                       r.pair_distances(), r.fft, r.p2o, c, dt, K,
+                       r.visible_patches, nthreads=8)
+    err = rel_err_within_db(out, ref)
+    print(f"[ds per_wall={per_wall}] groups {plan.n_groups} of {plan.Vd} "
+          f"entries, rel err {err:.3e}")
+    assert err <= EXCHANGE_RTOL
+
+
+def test_ds_plan_rejects_other_brdfs(ops):
+    """A BRDF that is not constant-plus-one-entry per row (directional
+    scattering) is reported as such; the grouped plan still applies."""
+    r = make_oracle(testing.shoebox_room_stub(3, 2.5, 2), .5)
+    rng = np.random.default_rng(0)
+    cc = samplings.hemisphere(samplings.sph_gaussian(sh_order=3))
+    freqs = np.array([500.])
+    n = cc.cartesian.shape[0]
+    br = B.create_from_directional_scattering(
+        cc, cc, FrequencyData(rng.uniform(0.01, 1.0, (n, n, 1)), freqs),
+        FrequencyData(np.array([0.1]), freqs))
+    r.set_wall_brdf(np.arange(r.n_walls), br.freq, cc.cartesian,
+                    cc.cartesian, freqs)
+    r.bake_geometry()
+    assert _ds_plan(ops, r, 343., 1 / 2000) is None
+
+
+def test_ds_plan_without_brdf(ops):
+    """No BRDF: D = 1, every entry diffuse with the form factor weight."""
+    r = make_oracle(testing.shoebox_room_stub(2, 2, 1), .5)
+    r.bake_geometry()
+    r.init_source_energy([.7, .6, .4])
+    T, c, dt, K = 200, 343., 1 / 2000, 5
+    plan = ops.diffuse_specular_plan(
+        r.visibility_matrix, r.patches_center, r.patches_area,
+        r.form_factors, r.p2o, None, r.patch_to_wall_ids, None, None, None,
+        1, 1, c, dt)
+    assert plan.n_groups == r.n_patches  # one diffuse group per row
+    st = ops.ExchangeState(r.n_patches, 1, 1, T)
+    st.init(r.energy_init_source, r.distance_patches_to_source, c, dt)
+    for _ in range(K):
+        st.step(plan)
+    ref = orc.exchange(T, r.energy_init_source, r.distance_patches_to_source,
+                       r.pair_distances(), r.fft, r.p2o, c, dt, K,
+                       r.visible_patches)
+    assert rel_err_within_db(_np(st.finalize()), ref) <= EXCHANGE_RTOL
