@@ -303,7 +303,9 @@ int spr_plan_ds_sort(int64_t P, int64_t D, int64_t max_row,
  * (diffuse), group keys m (specular direction) or D+1 (the diffuse group,
  * grp_start = row end), and rtab[D+2,B,DP] (unit vectors, a zero row and a
  * row of ones), to be used with spr_exchange_gather_ds and
- * spr_exchange_spread_grouped. */
+ * spr_exchange_spread_grouped.  Optional (may be NULL): krow[Vd] (int32,
+ * the BRDF row k of each entry) and wgt[B,Vd] (float64, the pair factor
+ * F~_ij exp(-air_b |u|) without the BRDF), which the adjoint uses. */
 int spr_plan_ds_fill(int64_t P, int64_t D, int64_t B, double c, double dt,
                      int64_t Vd, const int64_t *row_ptr, const int32_t *src,
                      const int32_t *key, const int64_t *grp_ptr,
@@ -314,7 +316,7 @@ int spr_plan_ds_fill(int64_t P, int64_t D, int64_t B, double c, double dt,
                      const double *a, const double *cs, int64_t *grp_start,
                      int32_t *grp_key, int32_t *rowid, int32_t *lag,
                      double *scal, double *dscal, double *rtab,
-                     void *stream);
+                     int32_t *krow, double *wgt, void *stream);
 
 /* Gather pass of the diffuse + specular plan: gsum rows of the specular
  * groups as spr_exchange_gather_grouped with scal, and the row's diffuse
@@ -371,6 +373,102 @@ int spr_collect(int64_t NR, int64_t P, int64_t D, int64_t B, int64_t T,
                 const double *centers, const int64_t *idx,
                 const double *factor, const double *air, double c, double dt,
                 int patchwise, float *out, void *stream);
+
+/* ---------------------------------------------------------------------
+ * Adjoint of the exchange (no counterpart in the reference, which has no
+ * gradients; the forward it differentiates is _energy_exchange,
+ * RadiosityFast.py:1073-1145, on a diffuse + specular plan).  Tables
+ * a[NK,B] (diffuse constant) and cs[NK,B] (specular part) are the BRDF
+ * rows in the form spr_brdf_diffuse_specular returns.
+ * ------------------------------------------------------------------- */
+
+/* Plan weights from tables: scal[b,e] = wgt[b,e] cs[krow_e, b],
+ * dscal[b,e] = wgt[b,e] a[krow_e, b] (krow, wgt from spr_plan_ds_fill). */
+int spr_ds_weights(int64_t B, int64_t Vd, const int32_t *krow,
+                   const double *wgt, const double *a, const double *cs,
+                   double *scal, double *dscal, void *stream);
+
+/* Injection from tables (_add_directional, RadiosityFast.py:988-1034):
+ * k0[P] (int32) = BRDF row of each patch for the direction of the source,
+ * e0dir[P,D,B] = energy0[P,B] (a + cs e_m)[k0, b] with m = spec_dir[k0].
+ * energy0 is spr_source_energy's output before the BRDF. */
+int spr_ds_injection(int64_t P, int64_t D, int64_t B, const double *source,
+                     const double *centers, const int64_t *wall_ids,
+                     int64_t S, const int64_t *brdf_index,
+                     const double *in_dirs, const double *energy0,
+                     const double *a, const double *cs,
+                     const int32_t *spec_dir, int32_t *k0, double *e0dir,
+                     void *stream);
+
+/* Source-major view of a diffuse + specular plan.  Pass 1: trow_ptr[P+1]
+ * (int64) = exclusive scan of the entries per source (rowid / D). */
+int spr_plan_ds_transpose_rows(int64_t P, int64_t D, const int64_t *row_ptr,
+                               const int32_t *rowid, int64_t *trow_ptr,
+                               void *stream);
+
+/* Pass 2 (max_trow = longest source row, at most 8192): for transposed
+ * entry q of source i, sorted by (direction of the source row, destination):
+ * te[q] (forward entry index), tj[q] (destination), tlag[q] (delay, bit 30
+ * on the last entry of a direction group), tm[q] (specular direction of
+ * the entry, 0 if none), tgptr[P*D+1] (int64: first entry of source i with
+ * direction >= d at i*D + d).  cursor[P] (int64) and tkey[Vd] (int32) are
+ * workspace; key[Vd] is the plan's entry key (specular direction or D). */
+int spr_plan_ds_transpose_fill(int64_t P, int64_t D, int64_t Vd,
+                               int64_t max_trow, const int64_t *row_ptr,
+                               const int32_t *rowid, const int32_t *lag,
+                               const int32_t *key, const int64_t *trow_ptr,
+                               int64_t *cursor, int32_t *te, int32_t *tkey,
+                               int32_t *tj, int32_t *tlag, int32_t *tm,
+                               int64_t *tgptr, void *stream);
+
+/* Transposed weights: twa[b,q] = dscal[b, te[q]], twc[b,q] = scal[b,te[q]]. */
+int spr_ds_transposed_weights(int64_t B, int64_t Vd, const int32_t *te,
+                              const double *scal, const double *dscal,
+                              double *twa, double *twc, void *stream);
+
+/* Adjoint of spr_collect (mono): lam[P,D,B,T] (float32, zero-filled here)
+ * = dL/d(total histogram) for grad[NR,B,T] = dL/d(receiver histograms);
+ * the other arguments as for spr_collect. */
+int spr_collect_adjoint(int64_t NR, int64_t P, int64_t D, int64_t B,
+                        int64_t T, const float *grad, const double *points,
+                        const double *centers, const int64_t *idx,
+                        const double *factor, const double *air, double c,
+                        double dt, float *lam, void *stream);
+
+/* musum[P,B,T] = sum over d of mu[P,D,B,T] (float64 sum). */
+int spr_direction_sum(int64_t P, int64_t D, int64_t B, int64_t T,
+                      const float *mu, float *musum, void *stream);
+
+/* One backward order: muout = lam + A^T munext (the transposed exchange),
+ * with musum = spr_direction_sum(munext) and ek the forward order-k
+ * histogram; adds each transposed entry's table-gradient dot products to
+ * ga[2,B,Vd] (float64): ga[0,b,q] += sum_t musum[j,b,t+n] ek[i,d,b,t],
+ * ga[1,b,q] += sum_t munext[j,m,b,t+n] ek[i,d,b,t].  muout must not alias
+ * an input.  T < 2^29. */
+int spr_exchange_adjoint_ds(int64_t P, int64_t D, int64_t B, int64_t T,
+                            int64_t Vd, const int64_t *tgptr,
+                            const int32_t *tj, const int32_t *tlag,
+                            const int32_t *tm, const double *twa,
+                            const double *twc, const float *lam,
+                            const float *musum, const float *munext,
+                            const float *ek, float *muout, double *ga,
+                            void *stream);
+
+/* Table gradients from ga: gA[k,b] += wgt[b,e] ga[0,b,q] and gC[k,b] +=
+ * wgt[b,e] ga[1,b,q] for e = te[q], k = krow[e] (float64 atomics). */
+int spr_ds_table_grad(int64_t B, int64_t Vd, const int32_t *te,
+                      const int32_t *krow, const double *wgt,
+                      const double *ga, double *gA, double *gC,
+                      void *stream);
+
+/* Injection part of the table gradients, from mu0 = dL/dE_0[P,D,B,T]:
+ * gA[k0_p,b] += energy0[p,b] sum_d mu0[p,d,b,n_p], gC[k0_p,b] +=
+ * energy0[p,b] mu0[p,m,b,n_p] with n_p = int(d0[p]/c/dt). */
+int spr_ds_injection_grad(int64_t P, int64_t D, int64_t B, int64_t T,
+                          const double *energy0, const double *d0, double c,
+                          double dt, const int32_t *k0,
+                          const int32_t *spec_dir, const float *mu0,
+                          double *gA, double *gC, void *stream);
 
 #ifdef __cplusplus
 }

@@ -95,9 +95,28 @@ SYMBOLS = {
     "spr_plan_ds_fill": (_int, [_i64, _i64, _i64, _dbl, _dbl, _i64, _vp, _vp,
                                 _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i64,
                                 _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
-                                _vp, _vp, _vp]),
+                                _vp, _vp, _vp, _vp, _vp]),
     "spr_exchange_gather_ds": (_int, [_i64, _i64, _i64, _i64, _vp, _vp, _vp,
                                       _vp, _vp, _vp, _vp, _vp, _vp]),
+    "spr_ds_weights": (_int, [_i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "spr_ds_injection": (_int, [_i64, _i64, _i64, _vp, _vp, _vp, _i64, _vp,
+                                _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "spr_plan_ds_transpose_rows": (_int, [_i64, _i64, _vp, _vp, _vp, _vp]),
+    "spr_plan_ds_transpose_fill": (_int, [_i64, _i64, _i64, _i64, _vp, _vp,
+                                          _vp, _vp, _vp, _vp, _vp, _vp, _vp,
+                                          _vp, _vp, _vp, _vp]),
+    "spr_ds_transposed_weights": (_int, [_i64, _i64, _vp, _vp, _vp, _vp, _vp,
+                                         _vp]),
+    "spr_collect_adjoint": (_int, [_i64, _i64, _i64, _i64, _i64, _vp, _vp,
+                                   _vp, _vp, _vp, _vp, _dbl, _dbl, _vp, _vp]),
+    "spr_direction_sum": (_int, [_i64, _i64, _i64, _i64, _vp, _vp, _vp]),
+    "spr_exchange_adjoint_ds": (_int, [_i64, _i64, _i64, _i64, _i64, _vp, _vp,
+                                       _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
+                                       _vp, _vp, _vp]),
+    "spr_ds_table_grad": (_int, [_i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp,
+                                 _vp]),
+    "spr_ds_injection_grad": (_int, [_i64, _i64, _i64, _i64, _vp, _vp, _dbl,
+                                     _dbl, _vp, _vp, _vp, _vp, _vp, _vp]),
     "spr_energy_exchange_workspace": (_i64, [_i64, _i64, _i64, _i64, _i64]),
     "spr_energy_exchange": (_int, [_i64, _i64, _i64, _i64, _vp, _vp, _vp,
                                    _vp, _vp, _dbl, _dbl, _i64, _i64, _vp,

@@ -1,0 +1,274 @@
+"""Gradients of receiver histograms with respect to the BRDF tables.
+
+The reference computes no gradients; this module differentiates its
+forward path -- injection (``RadiosityFast.py:988-1034``), the energy
+exchange (``:1073-1145``) and the mono receiver collection (``:570-602``,
+``:1148-1185``) -- for scenes whose BRDF rows are a diffuse constant plus
+one specular entry (what ``create_from_scattering``, ``brdf.py:8-130``,
+produces).  Such a row is ``A[k, b] * 1 + C[k, b] * e_m(k)``; the forward
+runs on the diffuse + specular exchange plan with weights rebuilt from
+``(A, C)``, and the backward is the hand-written adjoint
+(``spr_exchange_adjoint_ds``): the transposed exchange, order by order from
+the last, with the forward order histograms recomputed from checkpoints
+kept every ``checkpoint_every`` orders.
+
+:class:`ExchangeAdjoint` holds one scene/source/receiver set;
+:func:`receiver_histograms` is its ``torch.autograd.Function`` entry and
+:func:`scattering_tables` builds ``(A, C)`` from absorption and scattering
+coefficients differentiably, so gradients reach those coefficients.
+"""
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from . import _hip, ops
+from ._hip import call, ptr, stream_ptr
+
+__all__ = ["ExchangeAdjoint", "receiver_histograms", "scattering_tables",
+           "specular_factors", "log_energy_loss"]
+
+
+def _i32(n, dev):
+    return torch.empty(max(int(n), 1), dtype=torch.int32, device=dev)
+
+
+class ExchangeAdjoint:
+    """Forward and adjoint of ``tables -> receiver histograms`` for one
+    baked scene, one source and a set of receivers.
+
+    ``radiosity``: a baked :class:`~sparrowpy_amd.radiosity.
+    DirectionalRadiosityFast` with BRDFs (of the diffuse + specular form)
+    and air attenuation set.  The histogram settings are those of
+    ``calculate_energy_exchange``.
+    """
+
+    def __init__(self, radiosity, source, receivers, speed_of_sound,
+                 time_resolution, duration, max_order, checkpoint_every=5):
+        r = radiosity
+        if r._bake is None:
+            raise RuntimeError("bake_geometry() must run first")
+        b, g = r._bake, r._geometry()
+        if b["brdf"] is None:
+            raise ValueError("the adjoint needs BRDF tables")
+        self.c, self.dt = float(speed_of_sound), float(time_resolution)
+        self.T = int(duration / time_resolution)
+        self.K = int(max_order)
+        self.every = max(1, int(checkpoint_every))
+        plan = ops.diffuse_specular_plan(
+            b["vis"], g["centers"], g["areas"], b["ff"], b["p2o"], b["air"],
+            g["wall_ids"], b["sources"], b["brdf"], b["brdf_index"], b["D"],
+            b["n_bins"], self.c, self.dt, keep_adjoint=True)
+        if plan is None:
+            raise ValueError("BRDF rows are not diffuse + specular")
+        self.plan, adj = plan, plan.adjoint
+        self.P, self.D, self.B = plan.P, plan.D, plan.B
+        self.NK = int(np.shape(b["brdf"])[0]) * adj["S"]
+        dev = g["centers"].device
+        self.dev = dev
+        tables = ops.diffuse_specular_tables(b["brdf"], self.D, self.B,
+                                             self.NK)
+        self.A0, self.C0, self.spec_dir = tables
+        self._transpose()
+        # injection before the BRDF, and the source-row map
+        pos = np.asarray(source.cartesian if hasattr(source, "cartesian")
+                         else source, dtype=float).reshape(3)
+        self.source = ops.as_dev(pos, torch.float64)
+        vis = ops.check_point2patch_visibility(
+            pos, g["centers"], g["walls_normal"], g["walls_points"])[0]
+        e0, _, d0 = ops.source_energy(
+            pos, g["centers"], g["points"], vis, b["air"], self.B,
+            g["wall_ids"], b["sources"], b["brdf"], b["brdf_index"], self.D)
+        self.energy0, self.d0 = e0, d0
+        self.k0 = _i32(self.P, dev)
+        self.e0dir = torch.empty((self.P, self.D, self.B), dtype=torch.float64,
+                                 device=dev)
+        self.S = adj["S"]
+        self._geom = g
+        self.brdf_index = ops.as_dev(b["brdf_index"], torch.int64)
+        self.in_dirs = ops.as_dev(b["sources"], torch.float64)
+        # receivers
+        self.rpts, _, self.rfactor, self.ridx = r._receiver_terms(receivers)
+        self.air = ops.as_dev(b["air"] if b["air"] is not None
+                              else np.zeros(self.B), torch.float64)
+        self.R = self.rpts.shape[0]
+
+    # -- plan views ----------------------------------------------------------
+    def _transpose(self):
+        p, adj, dev = self.plan, self.plan.adjoint, self.dev
+        P, D, Vd = self.P, self.D, self.plan.Vd
+        self.trow_ptr = torch.empty(P + 1, dtype=torch.int64, device=dev)
+        call("spr_plan_ds_transpose_rows", P, D, ptr(p.row_ptr),
+             ptr(p.rowid), ptr(self.trow_ptr), stream_ptr())
+        cnt = self.trow_ptr[1:] - self.trow_ptr[:-1]
+        max_trow = int(cnt.max().item()) if P else 0
+        cursor = torch.empty(P, dtype=torch.int64, device=dev)
+        self.te, tkey = _i32(Vd, dev), _i32(Vd, dev)
+        self.tj, self.tlag, self.tm = _i32(Vd, dev), _i32(Vd, dev), \
+            _i32(Vd, dev)
+        self.tgptr = torch.empty(P * D + 1, dtype=torch.int64, device=dev)
+        call("spr_plan_ds_transpose_fill", P, D, Vd, max_trow,
+             ptr(p.row_ptr), ptr(p.rowid), ptr(p.lag), ptr(adj["key"]),
+             ptr(self.trow_ptr), ptr(cursor), ptr(self.te), ptr(tkey),
+             ptr(self.tj), ptr(self.tlag), ptr(self.tm), ptr(self.tgptr),
+             stream_ptr())
+
+    def _set_tables(self, A, C):
+        p, adj = self.plan, self.plan.adjoint
+        A = ops.as_dev(A, torch.float64)
+        C = ops.as_dev(C, torch.float64)
+        if A.shape != (self.NK, self.B) or C.shape != (self.NK, self.B):
+            raise ValueError(f"tables must be ({self.NK}, {self.B})")
+        call("spr_ds_weights", self.B, p.Vd, ptr(adj["krow"]),
+             ptr(adj["wgt"]), ptr(A), ptr(C), ptr(p.scal), ptr(p.dscal),
+             stream_ptr())
+        g = self._geom
+        call("spr_ds_injection", self.P, self.D, self.B, ptr(self.source),
+             ptr(g["centers"]), ptr(g["wall_ids"]), self.S,
+             ptr(self.brdf_index), ptr(self.in_dirs), ptr(self.energy0),
+             ptr(A), ptr(C), ptr(self.spec_dir), ptr(self.k0),
+             ptr(self.e0dir), stream_ptr())
+
+    # -- forward ----------------------------------------------------------------
+    def _order(self, state_prev, out, gsum):
+        """One order without the running total: out = A state_prev."""
+        p = self.plan
+        call("spr_exchange_gather_ds", self.P, self.B, self.T, p.Vd,
+             ptr(p.row_ptr), ptr(p.grp_ptr), ptr(p.rowid), ptr(p.lag),
+             ptr(p.scal), ptr(p.dscal), ptr(state_prev), ptr(gsum),
+             stream_ptr())
+        call("spr_exchange_spread_grouped", self.P, self.D, self.B, self.T,
+             ptr(p.grp_ptr), ptr(p.grp_key), ptr(p.rtab), ptr(gsum),
+             ptr(out), None, None, stream_ptr())
+
+    def forward(self, A, C):
+        """Receiver histograms (float32 [R, B, T]) for tables (A, C);
+        keeps the order checkpoints for :meth:`backward`."""
+        self._set_tables(A, C)
+        st = ops.ExchangeState(self.P, self.D, self.B, self.T, self.dev)
+        st.init(self.e0dir, self.d0, self.c, self.dt)
+        self.checkpoints = {0: st.prev.clone()}
+        for k in range(self.K):
+            st.step(self.plan)
+            if (k + 1) % self.every == 0 and k + 1 < self.K:
+                self.checkpoints[k + 1] = st.prev.clone()
+        st.check()
+        total = st.finalize(out=st.hi)
+        self._gsum = st.gsum if st.gsum is not None else torch.empty(
+            (max(self.plan.n_groups, 1), self.B, self.T), dtype=torch.float32,
+            device=self.dev)
+        H = ops.collect(total, self.rpts, self._geom["centers"], self.ridx,
+                        self.rfactor, self.air, self.c, self.dt)
+        del st, total
+        return H
+
+    # -- backward ---------------------------------------------------------------
+    def backward(self, grad_hist):
+        """(dL/dA, dL/dC) (float64 [NK, B]) for dL/dH = ``grad_hist``
+        ([R, B, T]) at the tables of the last :meth:`forward`."""
+        P, D, B, T, Vd = self.P, self.D, self.B, self.T, self.plan.Vd
+        dev, p, adj = self.dev, self.plan, self.plan.adjoint
+        gH = ops.as_dev(grad_hist, torch.float32).contiguous()
+        shape = (P, D, B, T)
+        lam = torch.empty(shape, dtype=torch.float32, device=dev)
+        call("spr_collect_adjoint", self.R, P, D, B, T, ptr(gH),
+             ptr(self.rpts), ptr(self._geom["centers"]), ptr(self.ridx),
+             ptr(self.rfactor), ptr(self.air), self.c, self.dt, ptr(lam),
+             stream_ptr())
+        twa = torch.empty((B, max(Vd, 1)), dtype=torch.float64, device=dev)
+        twc = torch.empty_like(twa)
+        call("spr_ds_transposed_weights", B, Vd, ptr(self.te), ptr(p.scal),
+             ptr(p.dscal), ptr(twa), ptr(twc), stream_ptr())
+        ga = torch.zeros((2, B, max(Vd, 1)), dtype=torch.float64, device=dev)
+        musum = torch.empty((P, B, T), dtype=torch.float32, device=dev)
+        mu = lam.clone()                     # mu_K
+        mu_out = torch.empty_like(lam)
+        starts = sorted(self.checkpoints)
+        for s in reversed(starts):
+            end = min(s + self.every, self.K)
+            states = [self.checkpoints[s]]   # E_s .. E_{end-1}
+            for k in range(s, end - 1):
+                nxt = torch.empty_like(lam)
+                self._order(states[-1], nxt, self._gsum)
+                states.append(nxt)
+            for k in range(end - 1, s - 1, -1):
+                call("spr_direction_sum", P, D, B, T, ptr(mu), ptr(musum),
+                     stream_ptr())
+                call("spr_exchange_adjoint_ds", P, D, B, T, Vd,
+                     ptr(self.tgptr), ptr(self.tj), ptr(self.tlag),
+                     ptr(self.tm), ptr(twa), ptr(twc), ptr(lam), ptr(musum),
+                     ptr(mu), ptr(states[k - s]), ptr(mu_out), ptr(ga),
+                     stream_ptr())
+                mu, mu_out = mu_out, mu
+            del states
+        gA = torch.zeros((self.NK, B), dtype=torch.float64, device=dev)
+        gC = torch.zeros_like(gA)
+        call("spr_ds_injection_grad", P, D, B, T, ptr(self.energy0),
+             ptr(self.d0), self.c, self.dt, ptr(self.k0), ptr(self.spec_dir),
+             ptr(mu), ptr(gA), ptr(gC), stream_ptr())
+        call("spr_ds_table_grad", B, Vd, ptr(self.te), ptr(adj["krow"]),
+             ptr(adj["wgt"]), ptr(ga), ptr(gA), ptr(gC), stream_ptr())
+        return gA, gC
+
+    def release(self):
+        """Drop the checkpoints of the last forward."""
+        self.checkpoints = {}
+
+
+class _Histograms(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, A, C, adj):
+        ctx.adj = adj
+        return adj.forward(A.detach(), C.detach()).to(torch.float64)
+
+    @staticmethod
+    def backward(ctx, grad):
+        gA, gC = ctx.adj.backward(grad)
+        ctx.adj.release()
+        return gA, gC, None
+
+
+def receiver_histograms(adj, A, C):
+    """Receiver histograms [R, B, T] (float64 tensor) as a differentiable
+    function of the tables ``A``, ``C`` ([NK, B] float64 device tensors)."""
+    return _Histograms.apply(A, C, adj)
+
+
+def specular_factors(directions):
+    """Per incoming direction s of a BRDF direction set, the specular value
+    1 / (cos(theta_s) w_m) that ``create_from_scattering`` puts at the
+    specular direction m(s) (``brdf.py:104-118``), and m(s)."""
+    from .brdf import create_from_scattering
+    from .coordinates import FrequencyData
+    zero = create_from_scattering(
+        directions, directions, FrequencyData(np.zeros((1, 1)), [1000.]),
+        FrequencyData(np.zeros((1, 1)), [1000.])).freq[..., 0]
+    m = np.argmax(zero, axis=1)
+    return zero[np.arange(zero.shape[0]), m], m
+
+
+def scattering_tables(alpha, scattering, spec_value):
+    """Diffuse and specular tables of ``create_from_scattering`` BRDFs:
+    for BRDF n with absorption alpha[n, b] and scattering s[n, b] and
+    incoming direction s_in, row k = n * S + s_in holds
+    A = (1 - alpha) s / pi and C = (1 - alpha)(1 - s) spec_value[s_in]
+    (times pi as the radiosity class stores them: A * pi, C * pi).
+    Differentiable in alpha and scattering ([NBRDF, B] tensors)."""
+    sv = torch.as_tensor(np.asarray(spec_value), dtype=alpha.dtype,
+                         device=alpha.device)
+    keep = 1.0 - alpha                                  # [N, B]
+    A = (keep * scattering)[:, None, :].expand(-1, sv.shape[0], -1)
+    C = (keep * (1.0 - scattering))[:, None, :] * sv[None, :, None] * np.pi
+    return A.reshape(-1, alpha.shape[1]), C.reshape(-1, alpha.shape[1])
+
+
+def log_energy_loss(H, target, dynamic_range_db=60.0):
+    """Sum of squared log10-energy differences over the bins of each
+    target histogram within ``dynamic_range_db`` of its peak."""
+    t = torch.as_tensor(target, dtype=H.dtype, device=H.device)
+    peak = t.amax(dim=-1, keepdim=True)
+    mask = (t >= peak * 10 ** (-dynamic_range_db / 10)) & (t > 0)
+    Hm = torch.where(mask, H, torch.ones_like(H))
+    tm = torch.where(mask, t, torch.ones_like(t))
+    d = torch.log10(Hm) - torch.log10(tm)
+    return (d * d * mask).sum()

@@ -1,0 +1,401 @@
+// Adjoint of the energy exchange on a diffuse + specular plan.
+//
+// Forward (RadiosityFast.py:1073-1145): E_{k+1}[j,d,b,t] = sum over entries
+// e = (i -> j) of F~_e[d,b] * E_k[i, dir_e, b, t - n_e], total = sum_k E_k,
+// with F~_e[d,b] = w_eb (A[k_e,b] + C[k_e,b] [d = m_e]) (A: diffuse
+// constant, C: specular part of BRDF row k_e; w_eb the pair factor).
+// For a loss L(total) with lambda = dL/dtotal:
+//   mu_K = lambda,  mu_k = lambda + A^T mu_{k+1},
+//   (A^T mu)[i,d',b,t] = sum_{e: i -> j, dir_e = d'}
+//        w_eb (A_b sum_d mu[j,d,b,t+n_e] + C_b mu[j,m_e,b,t+n_e]),
+//   dL/d(w A)_eb += sum_t (sum_d mu_{k+1}[j,d,b,t]) E_k[i,d',b,t-n_e],
+//   dL/d(w C)_eb += sum_t mu_{k+1}[j,m_e,b,t] E_k[i,d',b,t-n_e],
+// and dL/dE_0 = mu_0, which reaches the tables through the injection
+// E_0dir[i,:,b] = E0[i,b] (A + C e_m)[k0_i, b] (RadiosityFast.py:988-1034).
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "spr_common.hpp"
+#include "spr_exchange.hpp"
+#include "spr_util.hpp"
+
+namespace spr {
+namespace {
+
+// scal[b,e] = wgt[b,e] C[krow_e, b], dscal[b,e] = wgt[b,e] A[krow_e, b]
+__global__ void ds_weights_kernel(int64_t B, int64_t Vd,
+                                  const int32_t *__restrict__ krow,
+                                  const double *__restrict__ wgt,
+                                  const double *__restrict__ ta,
+                                  const double *__restrict__ tc,
+                                  double *__restrict__ scal,
+                                  double *__restrict__ dscal) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= B * Vd) return;
+  const int64_t b = q / Vd, e = q % Vd;
+  const int64_t k = krow[e];
+  scal[q] = wgt[q] * tc[k * B + b];
+  dscal[q] = wgt[q] * ta[k * B + b];
+}
+
+// Transposed weights: twa[b,q] = dscal[b, te[q]], twc[b,q] = scal[b, te[q]].
+__global__ void t_weights_kernel(int64_t B, int64_t Vd,
+                                 const int32_t *__restrict__ te,
+                                 const double *__restrict__ scal,
+                                 const double *__restrict__ dscal,
+                                 double *__restrict__ twa,
+                                 double *__restrict__ twc) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= B * Vd) return;
+  const int64_t b = q / Vd, p = q % Vd;
+  const int64_t e = te[p];
+  twa[q] = dscal[b * Vd + e];
+  twc[q] = scal[b * Vd + e];
+}
+
+// Injection from the tables: k0[p] = BRDF row of patch p for the direction
+// of the source, e0dir[p,d,b] = energy0[p,b] (A[k0,b] + C[k0,b] [d = m]).
+__global__ void ds_injection_kernel(
+    int64_t P, int64_t D, int64_t B, const double *__restrict__ source,
+    const double *__restrict__ centers, const int64_t *__restrict__ wall_ids,
+    int64_t S, const int64_t *__restrict__ brdf_index,
+    const double *__restrict__ in_dirs, const double *__restrict__ energy0,
+    const double *__restrict__ ta, const double *__restrict__ tc,
+    const int32_t *__restrict__ spec_dir, int32_t *__restrict__ k0,
+    double *__restrict__ e0dir) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= P) return;
+  int64_t k = 0;
+  if (in_dirs) {
+    d3 u = load3(source) - load3(centers + 3 * p);
+    u = div(u, norm(u));
+    const int64_t w = wall_ids[p];
+    k = brdf_index[w] * S + nearest_direction(in_dirs + w * S * 3, S, u);
+  }
+  k0[p] = (int32_t)k;
+  const int64_t m = spec_dir[k];
+  for (int64_t b = 0; b < B; ++b) {
+    const double e = energy0[p * B + b];
+    for (int64_t d = 0; d < D; ++d)
+      e0dir[(p * D + d) * B + b] =
+          e * (d == m ? ta[k * B + b] + tc[k * B + b] : ta[k * B + b]);
+  }
+}
+
+// musum[j,b,t] = sum_d mu[j,d,b,t] (float64 sum, float32 result)
+__global__ void direction_sum_kernel(int64_t P, int64_t D, int64_t B,
+                                     int64_t T, const float *__restrict__ mu,
+                                     float *__restrict__ musum) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= P * B * T) return;
+  const int64_t t = q % T, b = (q / T) % B, j = q / (B * T);
+  double s = 0.0;
+  for (int64_t d = 0; d < D; ++d) s += (double)mu[((j * D + d) * B + b) * T + t];
+  musum[q] = (float)s;
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// One backward order: mu_out = lambda + A^T mu_next, and the entries' table
+// gradients ga[0,b,q] += sum_t musum[j,b,t+n] ek[i,d',b,t],
+// ga[1,b,q] += sum_t mu_next[j,m,b,t+n] ek[i,d',b,t] (q: transposed entry).
+// A workgroup takes one (source i, band, block of 256*R bins), sources
+// fastest; it walks the directions d' of source i, and for each the
+// entries of the direction group in chunks of C, the next chunk's loads
+// in flight while the current one is summed.  A bin t + n beyond the
+// histogram is an out-of-range buffer offset and reads 0.  The per-entry
+// dot products are reduced over each wave and added (float64 atomics) by
+// lanes 0 .. 2C-1.  Requires T < 2^29.
+template <int R, int C>
+__global__ __launch_bounds__(256) void adjoint_ds_kernel(
+    int D, int B, int T, int n_tblk, int64_t nblocks, int64_t Vd,
+    const int64_t *__restrict__ tgptr, const int32_t *__restrict__ tj,
+    const int32_t *__restrict__ tlag, const int32_t *__restrict__ tm,
+    const double *__restrict__ twa, const double *__restrict__ twc,
+    const float *__restrict__ lam, const float *__restrict__ musum,
+    const float *__restrict__ munext, const float *__restrict__ ek,
+    float *__restrict__ muout, double *__restrict__ ga) {
+  constexpr int BS = 256;
+  int64_t i;
+  int b, tb;
+  dest_major_tile(xcd_tile(blockIdx.x, nblocks), nblocks / (B * n_tblk),
+                  n_tblk, i, b, tb);
+  const int tl = tb * (BS * R) + (int)threadIdx.x;
+  const int lane = (int)threadIdx.x & 63;
+  const uint32_t row_bytes = (uint32_t)T * 4u;
+  const double *wa = twa + (int64_t)b * Vd, *wc = twc + (int64_t)b * Vd;
+  double *gda = ga + (int64_t)b * Vd, *gdc = ga + ((int64_t)B + b) * Vd;
+
+  for (int d = 0; d < D; ++d) {
+    const int64_t q0 = tgptr[i * D + d], q1 = tgptr[i * D + d + 1];
+    const int64_t roff = ((i * D + d) * B + b) * (int64_t)T;
+    const __amdgpu_buffer_rsrc_t rl = make_rsrc(lam + roff, row_bytes);
+    const __amdgpu_buffer_rsrc_t ro = make_rsrc(muout + roff, row_bytes);
+    float lv[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+      lv[r] = buffer_load(rl, (uint32_t)(tl + BS * r) * 4u);
+    if (q1 == q0) {
+#pragma unroll
+      for (int r = 0; r < R; ++r)
+        buffer_store(lv[r], ro, (uint32_t)(tl + BS * r) * 4u);
+      continue;
+    }
+    const __amdgpu_buffer_rsrc_t re = make_rsrc(ek + roff, row_bytes);
+    float ev[R];
+    double acc[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      ev[r] = buffer_load(re, (uint32_t)(tl + BS * r) * 4u);
+      acc[r] = 0.0;
+    }
+    auto fetch = [&](int64_t qq, float (&x)[C][R], float (&y)[C][R]) {
+#pragma unroll
+      for (int c = 0; c < C; ++c) {
+        const bool in = qq + c < q1;
+        const int64_t qc = in ? qq + c : q1 - 1;
+        const int64_t j = tj[qc], m = tm[qc];
+        const int n = tlag[qc] & kLagMask;
+        const __amdgpu_buffer_rsrc_t rx = make_rsrc(
+            musum + (j * B + b) * (int64_t)T, in ? row_bytes : 0u);
+        const __amdgpu_buffer_rsrc_t ry = make_rsrc(
+            munext + ((j * D + m) * B + b) * (int64_t)T, in ? row_bytes : 0u);
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          const uint32_t off = (uint32_t)(tl + BS * r + n) * 4u;
+          x[c][r] = buffer_load(rx, off);
+          y[c][r] = buffer_load(ry, off);
+        }
+      }
+    };
+    auto consume = [&](int64_t qq, const float (&x)[C][R],
+                       const float (&y)[C][R]) {
+      double pa[C], pc[C];
+#pragma unroll
+      for (int c = 0; c < C; ++c) {
+        pa[c] = pc[c] = 0.0;
+        if (qq + c < q1) {
+          const double a = wa[qq + c], s = wc[qq + c];
+#pragma unroll
+          for (int r = 0; r < R; ++r) {
+            const double xd = (double)x[c][r], yd = (double)y[c][r];
+            acc[r] = fma(a, xd, acc[r]);
+            acc[r] = fma(s, yd, acc[r]);
+            pa[c] = fma((double)ev[r], xd, pa[c]);
+            pc[c] = fma((double)ev[r], yd, pc[c]);
+          }
+        }
+      }
+#pragma unroll
+      for (int c = 0; c < C; ++c) {
+        pa[c] = wave_sum(pa[c]);
+        pc[c] = wave_sum(pc[c]);
+      }
+      if (lane < 2 * C) {
+        const int c = lane % C;
+        double v = 0.0;
+#pragma unroll
+        for (int cc = 0; cc < C; ++cc)
+          if (cc == c) v = lane < C ? pa[cc] : pc[cc];
+        if (qq + c < q1)
+          unsafeAtomicAdd((lane < C ? gda : gdc) + qq + c, v);
+      }
+    };
+    float xa[C][R], ya[C][R], xb[C][R], yb[C][R];
+    fetch(q0, xa, ya);
+    for (int64_t q = q0; q < q1; q += 2 * C) {
+      fetch(q + C, xb, yb);
+      consume(q, xa, ya);
+      fetch(q + 2 * C, xa, ya);
+      consume(q + C, xb, yb);
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+      buffer_store((float)((double)lv[r] + acc[r]), ro,
+                   (uint32_t)(tl + BS * r) * 4u);
+  }
+}
+
+// Table gradients from the entries' accumulated dot products:
+// gA[k,b] += wgt[b,e] ga[0,b,q], gC[k,b] += wgt[b,e] ga[1,b,q] with
+// e = te[q], k = krow[e].
+__global__ void table_grad_kernel(int64_t B, int64_t Vd,
+                                  const int32_t *__restrict__ te,
+                                  const int32_t *__restrict__ krow,
+                                  const double *__restrict__ wgt,
+                                  const double *__restrict__ ga,
+                                  double *__restrict__ gA,
+                                  double *__restrict__ gC) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= B * Vd) return;
+  const int64_t b = q / Vd, p = q % Vd;
+  const int64_t e = te[p];
+  const int64_t k = krow[e];
+  const double w = wgt[b * Vd + e];
+  unsafeAtomicAdd(gA + k * B + b, w * ga[q]);
+  unsafeAtomicAdd(gC + k * B + b, w * ga[B * Vd + q]);
+}
+
+// Injection gradient: dL/de0dir[p,d,b] = mu0[p,d,b,n_p] with n_p the
+// order-0 bin of patch p; gA[k0_p,b] += energy0[p,b] sum_d mu0[p,d,b,n_p],
+// gC[k0_p,b] += energy0[p,b] mu0[p,m,b,n_p].
+__global__ void injection_grad_kernel(
+    int64_t P, int64_t D, int64_t B, int64_t T,
+    const double *__restrict__ energy0, const double *__restrict__ d0,
+    double c, double dt, const int32_t *__restrict__ k0,
+    const int32_t *__restrict__ spec_dir, const float *__restrict__ mu0,
+    double *__restrict__ gA, double *__restrict__ gC) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= P * B) return;
+  const int64_t p = q / B, b = q % B;
+  const int64_t n = delay_floor(d0[p], c, dt);
+  if (n < 0 || n >= T) return;
+  const int64_t k = k0[p], m = spec_dir[k];
+  double s = 0.0, sm = 0.0;
+  for (int64_t d = 0; d < D; ++d) {
+    const double v = mu0[((p * D + d) * B + b) * T + n];
+    s += v;
+    if (d == m) sm = v;
+  }
+  const double e = energy0[p * B + b];
+  unsafeAtomicAdd(gA + k * B + b, e * s);
+  if (m < D) unsafeAtomicAdd(gC + k * B + b, e * sm);
+}
+
+}  // namespace
+}  // namespace spr
+
+using namespace spr;
+
+extern "C" int spr_ds_weights(int64_t B, int64_t Vd, const int32_t *krow,
+                              const double *wgt, const double *a,
+                              const double *cs, double *scal, double *dscal,
+                              void *stream) {
+  SPR_REQUIRE(B > 0 && Vd >= 0, "spr_ds_weights: bad sizes");
+  SPR_REQUIRE(krow && wgt && a && cs && scal && dscal,
+              "spr_ds_weights: null pointer");
+  if (Vd == 0) return SPR_OK;
+  hipLaunchKernelGGL(ds_weights_kernel, dim3((unsigned)ceil_div(B * Vd, 256)),
+                     dim3(256), 0, as_stream(stream), B, Vd, krow, wgt, a, cs,
+                     scal, dscal);
+  SPR_LAUNCH_CHECK();
+  return SPR_OK;
+}
+
+extern "C" int spr_ds_transposed_weights(int64_t B, int64_t Vd,
+                                         const int32_t *te,
+                                         const double *scal,
+                                         const double *dscal, double *twa,
+                                         double *twc, void *stream) {
+  SPR_REQUIRE(B > 0 && Vd >= 0, "spr_ds_transposed_weights: bad sizes");
+  SPR_REQUIRE(te && scal && dscal && twa && twc,
+              "spr_ds_transposed_weights: null pointer");
+  if (Vd == 0) return SPR_OK;
+  hipLaunchKernelGGL(t_weights_kernel, dim3((unsigned)ceil_div(B * Vd, 256)),
+                     dim3(256), 0, as_stream(stream), B, Vd, te, scal, dscal,
+                     twa, twc);
+  SPR_LAUNCH_CHECK();
+  return SPR_OK;
+}
+
+extern "C" int spr_ds_injection(int64_t P, int64_t D, int64_t B,
+                                const double *source, const double *centers,
+                                const int64_t *wall_ids, int64_t S,
+                                const int64_t *brdf_index,
+                                const double *in_dirs, const double *energy0,
+                                const double *a, const double *cs,
+                                const int32_t *spec_dir, int32_t *k0,
+                                double *e0dir, void *stream) {
+  SPR_REQUIRE(P > 0 && D > 0 && B > 0, "spr_ds_injection: bad sizes");
+  SPR_REQUIRE(source && centers && energy0 && a && cs && spec_dir && k0 &&
+                  e0dir,
+              "spr_ds_injection: null pointer");
+  SPR_REQUIRE(!in_dirs || (wall_ids && brdf_index && S > 0),
+              "spr_ds_injection: in_dirs needs wall_ids, brdf_index");
+  hipLaunchKernelGGL(ds_injection_kernel, dim3((unsigned)ceil_div(P, 128)),
+                     dim3(128), 0, as_stream(stream), P, D, B, source, centers,
+                     wall_ids, S, brdf_index, in_dirs, energy0, a, cs,
+                     spec_dir, k0, e0dir);
+  SPR_LAUNCH_CHECK();
+  return SPR_OK;
+}
+
+extern "C" int spr_direction_sum(int64_t P, int64_t D, int64_t B, int64_t T,
+                                 const float *mu, float *musum,
+                                 void *stream) {
+  SPR_REQUIRE(P > 0 && D > 0 && B > 0 && T > 0,
+              "spr_direction_sum: bad sizes");
+  SPR_REQUIRE(mu && musum, "spr_direction_sum: null pointer");
+  hipLaunchKernelGGL(direction_sum_kernel,
+                     dim3((unsigned)ceil_div(P * B * T, 256)), dim3(256), 0,
+                     as_stream(stream), P, D, B, T, mu, musum);
+  SPR_LAUNCH_CHECK();
+  return SPR_OK;
+}
+
+extern "C" int spr_exchange_adjoint_ds(
+    int64_t P, int64_t D, int64_t B, int64_t T, int64_t Vd,
+    const int64_t *tgptr, const int32_t *tj, const int32_t *tlag,
+    const int32_t *tm, const double *twa, const double *twc,
+    const float *lam, const float *musum, const float *munext,
+    const float *ek, float *muout, double *ga, void *stream) {
+  SPR_REQUIRE(P > 0 && D > 0 && D <= 64 && B > 0 && T > 0 && Vd >= 0,
+              "spr_exchange_adjoint_ds: bad sizes");
+  SPR_REQUIRE(T < ((int64_t)1 << 29),
+              "spr_exchange_adjoint_ds: histogram too long (T >= 2^29)");
+  SPR_REQUIRE(tgptr && tj && tlag && tm && twa && twc && lam && musum &&
+                  munext && ek && muout && ga,
+              "spr_exchange_adjoint_ds: null pointer");
+  SPR_REQUIRE(muout != munext && muout != lam && muout != ek,
+              "spr_exchange_adjoint_ds: muout aliases an input");
+  constexpr int R = 2, C = 4;
+  const int64_t nt = ceil_div(T, (int64_t)256 * R);
+  const int64_t nb = P * B * nt;
+  SPR_REQUIRE(nb < (int64_t)0x7fffffff, "spr_exchange_adjoint_ds: grid too "
+                                        "large");
+  hipLaunchKernelGGL((adjoint_ds_kernel<R, C>), dim3((unsigned)nb), dim3(256),
+                     0, as_stream(stream), (int)D, (int)B, (int)T, (int)nt,
+                     nb, Vd, tgptr, tj, tlag, tm, twa, twc, lam, musum, munext,
+                     ek, muout, ga);
+  SPR_LAUNCH_CHECK();
+  return SPR_OK;
+}
+
+extern "C" int spr_ds_table_grad(int64_t B, int64_t Vd, const int32_t *te,
+                                 const int32_t *krow, const double *wgt,
+                                 const double *ga, double *gA, double *gC,
+                                 void *stream) {
+  SPR_REQUIRE(B > 0 && Vd >= 0, "spr_ds_table_grad: bad sizes");
+  SPR_REQUIRE(te && krow && wgt && ga && gA && gC,
+              "spr_ds_table_grad: null pointer");
+  if (Vd == 0) return SPR_OK;
+  hipLaunchKernelGGL(table_grad_kernel, dim3((unsigned)ceil_div(B * Vd, 256)),
+                     dim3(256), 0, as_stream(stream), B, Vd, te, krow, wgt, ga,
+                     gA, gC);
+  SPR_LAUNCH_CHECK();
+  return SPR_OK;
+}
+
+extern "C" int spr_ds_injection_grad(int64_t P, int64_t D, int64_t B,
+                                     int64_t T, const double *energy0,
+                                     const double *d0, double c, double dt,
+                                     const int32_t *k0,
+                                     const int32_t *spec_dir,
+                                     const float *mu0, double *gA,
+                                     double *gC, void *stream) {
+  SPR_REQUIRE(P > 0 && D > 0 && B > 0 && T > 0 && c > 0 && dt > 0,
+              "spr_ds_injection_grad: bad sizes");
+  SPR_REQUIRE(energy0 && d0 && k0 && spec_dir && mu0 && gA && gC,
+              "spr_ds_injection_grad: null pointer");
+  hipLaunchKernelGGL(injection_grad_kernel,
+                     dim3((unsigned)ceil_div(P * B, 256)), dim3(256), 0,
+                     as_stream(stream), P, D, B, T, energy0, d0, c, dt, k0,
+                     spec_dir, mu0, gA, gC);
+  SPR_LAUNCH_CHECK();
+  return SPR_OK;
+}

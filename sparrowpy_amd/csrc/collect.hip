@@ -11,6 +11,7 @@
 // etc[k, idx, b, :] is read contiguously (circularly shifted by the delay).
 #include "spr_common.hpp"
 #include "spr_geom.hpp"
+#include "spr_util.hpp"
 
 namespace spr {
 namespace {
@@ -98,6 +99,33 @@ __global__ __launch_bounds__(256) void collect_patchwise_kernel(
   out[((r * P + k) * B + b) * T + t] = (float)(((double)row[ts] * g) * a);
 }
 
+// Adjoint of the mono collection: lam[k, idx_rk, b, s] += g_rk a_rkb
+// grad[r, b, (s + n_rk) mod T] for every receiver r (lam zero-filled
+// before); one thread per (patch, band, bin), receivers in order.
+__global__ __launch_bounds__(256) void collect_adjoint_kernel(
+    int64_t NR, int64_t P, int64_t D, int64_t B, int64_t T,
+    const float *__restrict__ grad, const double *__restrict__ points,
+    const double *__restrict__ centers, const int64_t *__restrict__ idx,
+    const double *__restrict__ factor, const double *__restrict__ air,
+    double c, double dt, float *__restrict__ lam) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= P * B * T) return;
+  const int64_t s = q % T, b = (q / T) % B, k = q / (B * T);
+  const d3 ck = load3(centers + 3 * k);
+  for (int64_t r = 0; r < NR; ++r) {
+    const double g = factor[r * P + k];
+    if (g == 0.0) continue;
+    const double dist = norm(ck - load3(points + 3 * r));
+    const int64_t n = delay_ceil(dist, c, dt);
+    const int64_t sh = ((n % T) + T) % T;
+    int64_t t = s + sh;
+    if (t >= T) t -= T;
+    const double a = exp(-air[b] * dist);
+    float *row = lam + ((k * D + idx[r * P + k]) * B + b) * T;
+    row[s] = (float)((double)row[s] + (g * a) * (double)grad[(r * B + b) * T + t]);
+  }
+}
+
 }  // namespace
 }  // namespace spr
 
@@ -144,6 +172,28 @@ extern "C" int spr_collect(int64_t NR, int64_t P, int64_t D, int64_t B,
                        dim3(256), 0, s, NR, P, D, B, T, (int)n_tblk, etc,
                        points, centers, idx, factor, air, c, dt, out);
   }
+  SPR_LAUNCH_CHECK();
+  return SPR_OK;
+}
+
+extern "C" int spr_collect_adjoint(int64_t NR, int64_t P, int64_t D,
+                                   int64_t B, int64_t T, const float *grad,
+                                   const double *points,
+                                   const double *centers, const int64_t *idx,
+                                   const double *factor, const double *air,
+                                   double c, double dt, float *lam,
+                                   void *stream) {
+  SPR_REQUIRE(NR > 0 && P > 0 && D > 0 && B > 0 && T > 0 && c > 0 && dt > 0,
+              "spr_collect_adjoint: bad sizes");
+  SPR_REQUIRE(grad && points && centers && idx && factor && air && lam,
+              "spr_collect_adjoint: null pointer");
+  hipStream_t s = as_stream(stream);
+  int rc = fill<float>(lam, P * D * B * T, 0.0f, s);
+  if (rc) return rc;
+  hipLaunchKernelGGL(collect_adjoint_kernel,
+                     dim3((unsigned)ceil_div(P * B * T, 256)), dim3(256), 0,
+                     s, NR, P, D, B, T, grad, points, centers, idx, factor,
+                     air, c, dt, lam);
   SPR_LAUNCH_CHECK();
   return SPR_OK;
 }

@@ -19,6 +19,7 @@
 #include "spr_common.hpp"
 #include "spr_fft.hpp"
 #include "spr_util.hpp"
+#include "spr_exchange.hpp"
 
 namespace spr {
 namespace {
@@ -160,11 +161,6 @@ __global__ void init_kernel(int64_t P, int64_t D, int64_t B, int64_t T,
 // one reflection order
 // ---------------------------------------------------------------------
 
-// Entry metadata of the grouped plan: the delay word carries, in bit 30,
-// the flag "last entry of its group" (delays are clamped below 2^30).
-constexpr int32_t kLastInGroup = 0x40000000;
-constexpr int32_t kLagMask = 0x3fffffff;
-
 // Gather of C consecutive plan entries starting at e (row ends at e_end):
 // the prev histogram row `row(e)` shifted by the entry's delay.  Metadata
 // loads are unconditional (the index is clamped into the row) so they can
@@ -291,47 +287,6 @@ __global__ __launch_bounds__(BS) void exchange_step_kernel(
   store_tile<DP, R, BS>(acc, D, B, T, j, b, t0, cur, tot_hi, tot_lo);
 }
 
-// Tile order of the grouped passes: destination fastest, then time block,
-// then band.  With the XCD-contiguous block mapping, the blocks resident on
-// one XCD at a time are neighbouring destinations (neighbouring patches of
-// a wall) in the same band and time window, which read nearly the same
-// source histogram segments -- so those reads are shared through L2/MALL.
-__device__ __forceinline__ void dest_major_tile(int64_t tile, int64_t P,
-                                                int n_tblk, int64_t &j,
-                                                int &b, int &tb) {
-  j = tile % P;
-  const int64_t rest = tile / P;
-  tb = (int)(rest % n_tblk);
-  b = (int)(rest / n_tblk);
-}
-
-// Raw buffer access with a gfx9 buffer descriptor: a load whose byte
-// offset lies outside [0, bytes) returns 0 and a store there is dropped,
-// with no branch and no memory access.  This lets every lane issue its
-// loads unconditionally, so the compiler can count them and keep the next
-// chunk's loads in flight while the current chunk is summed.
-constexpr int kBufferWord3 = 0x00020000;   // gfx9 descriptor word 3
-constexpr uint32_t kNoOffset = 0x80000000u;  // beyond every range used here
-
-__device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void *p,
-                                                            uint32_t bytes) {
-  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), 0,
-                                           (int)bytes, kBufferWord3);
-}
-
-__device__ __forceinline__ float buffer_load(__amdgpu_buffer_rsrc_t r,
-                                             uint32_t off) {
-  return __builtin_bit_cast(
-      float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)off, 0, 0));
-}
-
-__device__ __forceinline__ void buffer_store(float v,
-                                             __amdgpu_buffer_rsrc_t r,
-                                             uint32_t off) {
-  __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r,
-                                        (int)off, 0, 0);
-}
-
 // Grouped plan, pass 1 of an order: per group g of destination j's sources
 // (entries sharing one BRDF row), the weighted shifted sum
 //   gsum[g, b, t] = sum_{e in g} scal[b, e] * prev[rowid[e], b, t - lag[e]]
@@ -367,14 +322,6 @@ __device__ __forceinline__ EntryMeta load_meta(
   m.lag = in ? lag[ec] : kLagMask;  // past the row end: no bins, no flag
   m.w = in ? w[ec] : 0.0;
   return m;
-}
-
-__device__ __forceinline__ double readlane_f64(double v, int k) {
-  const int64_t u = __builtin_bit_cast(int64_t, v);
-  const int32_t lo = __builtin_amdgcn_readlane((int32_t)u, k);
-  const int32_t hi = __builtin_amdgcn_readlane((int32_t)(u >> 32), k);
-  return __builtin_bit_cast(double,
-                            ((int64_t)hi << 32) | (int64_t)(uint32_t)lo);
 }
 
 template <int R, int BS, int C>
@@ -563,7 +510,7 @@ struct SpreadShape {
   static constexpr int TILE = NWT * 64 * R;      // bins per workgroup
 };
 
-template <int DP>
+template <int DP, bool TOTAL>
 __global__ __launch_bounds__(256) void group_spread_kernel(
     int D, int B, int T, int n_tblk, int64_t nblocks,
     const int64_t *__restrict__ grp_ptr, const int32_t *__restrict__ grp_key,
@@ -638,9 +585,21 @@ __global__ __launch_bounds__(256) void group_spread_kernel(
   const int64_t base = j * D * dstride + (int64_t)b * T;
   const uint32_t span = (uint32_t)(((int64_t)(D - 1) * dstride + T) * 4);
   const __amdgpu_buffer_rsrc_t rc = make_rsrc(cur + base, span);
+  constexpr int NP = DW * R;  // (d, r) pairs of this lane
+  if (!TOTAL) {  // order histogram only (no running total)
+#pragma unroll
+    for (int p = 0; p < NP; ++p) {
+      const int d = d0 + p / R, r = p % R;
+      const int t = tl + 64 * r;
+      const uint32_t off = (d < D && t < T)
+                               ? (uint32_t)((int64_t)d * dstride + t) * 4u
+                               : kNoOffset;
+      buffer_store((float)acc[p / R][r], rc, off);
+    }
+    return;
+  }
   const __amdgpu_buffer_rsrc_t rh = make_rsrc(tot_hi + base, span);
   const __amdgpu_buffer_rsrc_t rl = make_rsrc(tot_lo + base, span);
-  constexpr int NP = DW * R;  // (d, r) pairs of this lane
 #pragma unroll
   for (int p0 = 0; p0 < NP; p0 += K) {
     uint32_t off[K];
@@ -735,9 +694,14 @@ int launch_spread(int64_t P, int64_t D, int64_t B, int64_t T,
   SPR_REQUIRE(nb < (int64_t)0x7fffffff, "exchange grid too large");
   SPR_REQUIRE(D * B * T < ((int64_t)1 << 29),
               "histogram too large for one patch (D*B*T >= 2^29)");
-  hipLaunchKernelGGL((group_spread_kernel<DP>), dim3((unsigned)nb),
-                     dim3(256), 0, s, (int)D, (int)B, (int)T, (int)nt, nb,
-                     grp_ptr, grp_key, rtab, gsum, cur, tot_hi, tot_lo);
+  if (tot_hi)
+    hipLaunchKernelGGL((group_spread_kernel<DP, true>), dim3((unsigned)nb),
+                       dim3(256), 0, s, (int)D, (int)B, (int)T, (int)nt, nb,
+                       grp_ptr, grp_key, rtab, gsum, cur, tot_hi, tot_lo);
+  else
+    hipLaunchKernelGGL((group_spread_kernel<DP, false>), dim3((unsigned)nb),
+                       dim3(256), 0, s, (int)D, (int)B, (int)T, (int)nt, nb,
+                       grp_ptr, grp_key, rtab, gsum, cur, tot_hi, tot_lo);
   SPR_LAUNCH_CHECK();
   return SPR_OK;
 }
@@ -746,20 +710,6 @@ int launch_spread(int64_t P, int64_t D, int64_t B, int64_t T,
 // grouped plan construction
 // ---------------------------------------------------------------------
 constexpr int kSortMax = 8192;  // longest destination row the LDS sort takes
-
-// BRDF row of entry e (source i -> destination j): brdf_index[w_i] * S +
-// s_ij (0 without a BRDF).
-__device__ __forceinline__ int64_t entry_brdf_row(
-    int64_t i, int64_t j, const double *__restrict__ centers,
-    const int64_t *__restrict__ wall_ids, int64_t S,
-    const int64_t *__restrict__ brdf_index,
-    const double *__restrict__ in_dirs) {
-  if (!in_dirs) return 0;
-  d3 u = load3(centers + 3 * i) - load3(centers + 3 * j);
-  u = div(u, norm(u));
-  const int64_t w = wall_ids[i];
-  return brdf_index[w] * S + nearest_direction(in_dirs + w * S * 3, S, u);
-}
 
 // Key of entry e: its BRDF row, or with spec_dir (diffuse + specular plan)
 // the specular direction of that row.
@@ -965,7 +915,8 @@ __global__ __launch_bounds__(256) void ds_entry_kernel(
     const double *__restrict__ in_dirs, const double *__restrict__ ta,
     const double *__restrict__ tc, int32_t *__restrict__ rowid,
     int32_t *__restrict__ lag, double *__restrict__ scal,
-    double *__restrict__ dscal) {
+    double *__restrict__ dscal, int32_t *__restrict__ krow,
+    double *__restrict__ wgt) {
   const int64_t j = blockIdx.x;
   const int64_t e1 = row_ptr[j + 1];
   for (int64_t e = row_ptr[j] + threadIdx.x; e < e1; e += blockDim.x) {
@@ -979,11 +930,13 @@ __global__ __launch_bounds__(256) void ds_entry_kernel(
                                      in_dirs);
     PairFactor pf = pair_factor(i, j, P, centers, areas, ff, nullptr, 1, 1, B,
                                 nullptr, nullptr, nullptr);
+    if (krow) krow[e] = (int32_t)k;
     for (int64_t b = 0; b < B; ++b) {
       double v = pf.f;
       if (air) v = v * exp(-air[b] * pf.distance);
       scal[b * Vd + e] = v * tc[k * B + b];
       dscal[b * Vd + e] = v * ta[k * B + b];
+      if (wgt) wgt[b * Vd + e] = v;
     }
   }
 }
@@ -998,6 +951,76 @@ __global__ void ds_rtab_kernel(int64_t D, int64_t B, int64_t DP,
   double v = 0.0;
   if (d < D) v = (k == D + 1) ? 1.0 : (k == d ? 1.0 : 0.0);
   rtab[q] = v;
+}
+
+// ---------------------------------------------------------------------
+// transposed (source-major) view of a diffuse + specular plan, for the
+// adjoint: row i lists the entries e = (i -> j) sorted by (direction of
+// the source row, destination)
+// ---------------------------------------------------------------------
+constexpr int kTransposeDstBits = 25;  // destination index bits of a key
+
+__global__ __launch_bounds__(256) void t_count_kernel(
+    int64_t P, int64_t D, const int64_t *__restrict__ row_ptr,
+    const int32_t *__restrict__ rowid, unsigned long long *__restrict__ cnt) {
+  const int64_t j = blockIdx.x;
+  for (int64_t e = row_ptr[j] + threadIdx.x; e < row_ptr[j + 1];
+       e += blockDim.x)
+    atomicAdd(&cnt[rowid[e] / D], 1ull);
+}
+
+__global__ __launch_bounds__(256) void t_scatter_kernel(
+    int64_t P, int64_t D, const int64_t *__restrict__ row_ptr,
+    const int32_t *__restrict__ rowid, const int64_t *__restrict__ trow_ptr,
+    unsigned long long *__restrict__ cursor, int32_t *__restrict__ te,
+    int32_t *__restrict__ tkey) {
+  const int64_t j = blockIdx.x;
+  for (int64_t e = row_ptr[j] + threadIdx.x; e < row_ptr[j + 1];
+       e += blockDim.x) {
+    const int64_t i = rowid[e] / D, d = rowid[e] % D;
+    const int64_t q = trow_ptr[i] + (int64_t)atomicAdd(&cursor[i], 1ull);
+    te[q] = (int32_t)e;
+    tkey[q] = (int32_t)((d << kTransposeDstBits) | j);
+  }
+}
+
+// After the per-row sort by key: destination, delay (bit 30 set on the last
+// entry of each direction group), specular direction (0 with a zero
+// weight when the entry has none), and the group pointers
+// tgptr[i*D + d] = first transposed entry of source i with direction >= d.
+__global__ __launch_bounds__(256) void t_fill_kernel(
+    int64_t P, int64_t D, const int64_t *__restrict__ trow_ptr,
+    const int32_t *__restrict__ te, const int32_t *__restrict__ tkey,
+    const int32_t *__restrict__ lag, const int32_t *__restrict__ key,
+    int32_t *__restrict__ tj, int32_t *__restrict__ tlag,
+    int32_t *__restrict__ tm, int64_t *__restrict__ tgptr) {
+  const int64_t i = blockIdx.x;
+  const int64_t q0 = trow_ptr[i], q1 = trow_ptr[i + 1];
+  const int32_t dmask = (1 << kTransposeDstBits) - 1;
+  for (int64_t q = q0 + threadIdx.x; q < q1; q += blockDim.x) {
+    const int32_t k = tkey[q];
+    const int32_t d = k >> kTransposeDstBits;
+    const bool last = q + 1 == q1 || (tkey[q + 1] >> kTransposeDstBits) != d;
+    const int32_t e = te[q];
+    tj[q] = k & dmask;
+    tlag[q] = (lag[e] & kLagMask) | (last ? kLastInGroup : 0);
+    tm[q] = key[e] < (int32_t)D ? key[e] : 0;
+  }
+  for (int64_t d = threadIdx.x; d <= D; d += blockDim.x) {
+    if (d == D) {
+      if (i == P - 1) tgptr[P * D] = q1;
+      continue;
+    }
+    int64_t lo = q0, hi = q1;  // first q with direction >= d
+    while (lo < hi) {
+      const int64_t mid = (lo + hi) / 2;
+      if ((tkey[mid] >> kTransposeDstBits) < d)
+        lo = mid + 1;
+      else
+        hi = mid;
+    }
+    tgptr[i * D + d] = lo;
+  }
 }
 
 inline size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
@@ -1279,7 +1302,8 @@ extern "C" int spr_plan_ds_fill(
     const int64_t *wall_ids, int64_t S, const int64_t *brdf_index,
     const double *in_dirs, const double *ta, const double *tc,
     int64_t *grp_start, int32_t *grp_key, int32_t *rowid, int32_t *lag,
-    double *scal, double *dscal, double *rtab, void *stream) {
+    double *scal, double *dscal, double *rtab, int32_t *krow, double *wgt,
+    void *stream) {
   const int64_t DP = spr_padded_directions(D);
   SPR_REQUIRE(DP > 0, "spr_plan_ds_fill: D=%lld unsupported (max 64)",
               (long long)D);
@@ -1300,7 +1324,7 @@ extern "C" int spr_plan_ds_fill(
     hipLaunchKernelGGL(ds_entry_kernel, dim3((unsigned)P), dim3(256), 0, s, P,
                        D, B, c, dt, Vd, row_ptr, src, key, centers, areas, ff,
                        p2o, air, wall_ids, S, brdf_index, in_dirs, ta, tc,
-                       rowid, lag, scal, dscal);
+                       rowid, lag, scal, dscal, krow, wgt);
     SPR_LAUNCH_CHECK();
   }
   const int64_t nr = (D + 2) * B * DP;
@@ -1324,6 +1348,69 @@ extern "C" int spr_exchange_gather_ds(
                        dscal, prev, gsum, as_stream(stream));
 }
 
+extern "C" int spr_plan_ds_transpose_rows(int64_t P, int64_t D,
+                                          const int64_t *row_ptr,
+                                          const int32_t *rowid,
+                                          int64_t *trow_ptr, void *stream) {
+  SPR_REQUIRE(P > 0 && D > 0, "spr_plan_ds_transpose_rows: bad sizes");
+  SPR_REQUIRE(row_ptr && rowid && trow_ptr,
+              "spr_plan_ds_transpose_rows: null pointer");
+  hipStream_t s = as_stream(stream);
+  int rc = fill<int64_t>(trow_ptr, P + 1, 0, s);
+  if (rc) return rc;
+  hipLaunchKernelGGL(t_count_kernel, dim3((unsigned)P), dim3(256), 0, s, P, D,
+                     row_ptr, rowid,
+                     reinterpret_cast<unsigned long long *>(trow_ptr));
+  SPR_LAUNCH_CHECK();
+  return scan_exclusive(trow_ptr, P, s);
+}
+
+extern "C" int spr_plan_ds_transpose_fill(
+    int64_t P, int64_t D, int64_t Vd, int64_t max_trow,
+    const int64_t *row_ptr, const int32_t *rowid, const int32_t *lag,
+    const int32_t *key, const int64_t *trow_ptr, int64_t *cursor,
+    int32_t *te, int32_t *tkey, int32_t *tj, int32_t *tlag, int32_t *tm,
+    int64_t *tgptr, void *stream) {
+  SPR_REQUIRE(P > 0 && D > 0 && Vd >= 0 && max_trow >= 0,
+              "spr_plan_ds_transpose_fill: bad sizes");
+  SPR_REQUIRE(P < ((int64_t)1 << kTransposeDstBits) && D <= 64,
+              "spr_plan_ds_transpose_fill: too many patches or directions");
+  SPR_REQUIRE(row_ptr && rowid && lag && key && trow_ptr && cursor && te &&
+                  tkey && tj && tlag && tm && tgptr,
+              "spr_plan_ds_transpose_fill: null pointer");
+  if (max_trow > kSortMax)
+    return fail(SPR_ERR_UNSUPPORTED,
+                "spr_plan_ds_transpose_fill: a source sees %lld patches, "
+                "the plan supports %d",
+                (long long)max_trow, kSortMax);
+  hipStream_t s = as_stream(stream);
+  int rc = fill<int64_t>(cursor, P, 0, s);
+  if (rc) return rc;
+  if (Vd > 0) {
+    hipLaunchKernelGGL(t_scatter_kernel, dim3((unsigned)P), dim3(256), 0, s,
+                       P, D, row_ptr, rowid, trow_ptr,
+                       reinterpret_cast<unsigned long long *>(cursor), te,
+                       tkey);
+    SPR_LAUNCH_CHECK();
+    int npad = 1;
+    while (npad < max_trow) npad <<= 1;
+    const size_t lds =
+        (size_t)npad * (sizeof(uint64_t) + sizeof(int32_t)) + sizeof(int32_t);
+    SPR_HIP(hipFuncSetAttribute((const void *)group_sort_kernel,
+                                hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)lds));
+    // sorts te by tkey within each transposed row (cursor: group counts,
+    // not used)
+    hipLaunchKernelGGL(group_sort_kernel, dim3((unsigned)P), dim3(1024), lds,
+                       s, P, trow_ptr, te, tkey, cursor, npad, (int32_t)-1, 0);
+    SPR_LAUNCH_CHECK();
+  }
+  hipLaunchKernelGGL(t_fill_kernel, dim3((unsigned)P), dim3(256), 0, s, P, D,
+                     trow_ptr, te, tkey, lag, key, tj, tlag, tm, tgptr);
+  SPR_LAUNCH_CHECK();
+  return SPR_OK;
+}
+
 extern "C" int spr_exchange_gather_grouped(
     int64_t P, int64_t B, int64_t T, int64_t Vd, const int64_t *row_ptr,
     const int64_t *grp_ptr, const int32_t *rowid, const int32_t *lag,
@@ -1345,8 +1432,10 @@ extern "C" int spr_exchange_spread_grouped(
               (long long)D);
   SPR_REQUIRE(P > 0 && B > 0 && T > 0 && T < 0x3fffffff,
               "spr_exchange_spread_grouped: bad sizes");
-  SPR_REQUIRE(grp_ptr && grp_key && rtab && gsum && cur && tot_hi && tot_lo,
+  SPR_REQUIRE(grp_ptr && grp_key && rtab && gsum && cur,
               "spr_exchange_spread_grouped: null pointer");
+  SPR_REQUIRE(!tot_hi == !tot_lo,
+              "spr_exchange_spread_grouped: tot_hi and tot_lo go together");
   hipStream_t s = as_stream(stream);
 #define SPR_SPREAD(DPV)                                                       \
   return launch_spread<DPV>(P, D, B, T, grp_ptr, grp_key, rtab, gsum, cur,    \

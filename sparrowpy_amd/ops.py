@@ -462,9 +462,10 @@ def diffuse_specular_tables(scattering, n_directions, n_bins, n_rows):
 def diffuse_specular_plan(vis, patches_center, patches_area, form_factors,
                           p2o, air_attenuation, patch_to_wall_ids, sources,
                           scattering, scattering_index, n_directions, n_bins,
-                          speed_of_sound, time_resolution):
+                          speed_of_sound, time_resolution, keep_adjoint=False):
     """GroupedPlan of the diffuse + specular form (see sparrow_hip.h), or
-    None if some BRDF row is not a constant plus one specular entry."""
+    None if some BRDF row is not a constant plus one specular entry.
+    ``keep_adjoint`` also keeps what the adjoint needs (``plan.adjoint``)."""
     c = _f64(patches_center)
     dev = c.device
     P = c.shape[0]
@@ -504,16 +505,27 @@ def diffuse_specular_plan(vis, patches_center, patches_area, form_factors,
     scal = torch.empty((B, max(Vd, 1)), dtype=torch.float64, device=dev)
     dscal = torch.empty((B, max(Vd, 1)), dtype=torch.float64, device=dev)
     rtab = torch.empty((D + 2, B, DP), dtype=torch.float64, device=dev)
+    krow = wgt = None
+    if keep_adjoint:
+        krow = torch.empty(max(Vd, 1), dtype=torch.int32, device=dev)
+        wgt = torch.empty((B, max(Vd, 1)), dtype=torch.float64, device=dev)
     call("spr_plan_ds_fill", P, D, B, float(speed_of_sound),
          float(time_resolution), Vd, ptr(row_ptr), ptr(src), ptr(key),
          ptr(grp_ptr), ptr(c), ptr(_f64(patches_area)),
          ptr(_f64(form_factors)), ptr(_i64(p2o)), ptr(_f64(air_attenuation)),
          ptr(w), S, ptr(bidx), ptr(sd), ptr(ta), ptr(tc), ptr(grp_start),
          ptr(grp_key), ptr(rowid), ptr(lag), ptr(scal), ptr(dscal),
-         ptr(rtab), stream_ptr())
-    return GroupedPlan(P, D, B, row_ptr, grp_ptr, grp_start, grp_key,
+         ptr(rtab), ptr(krow), ptr(wgt), stream_ptr())
+    plan = GroupedPlan(P, D, B, row_ptr, grp_ptr, grp_start, grp_key,
                        rowid[:Vd], lag[:Vd], scal[:, :Vd].contiguous(), rtab,
                        dscal[:, :Vd].contiguous())
+    if keep_adjoint:
+        # what the adjoint needs besides the plan: entry keys (specular
+        # direction, D for none), BRDF rows, pair factors, the tables
+        plan.adjoint = dict(key=key[:Vd], krow=krow[:Vd],
+                            wgt=wgt[:, :Vd].contiguous(), spec_dir=spec_dir,
+                            max_row=max_row, S=S)
+    return plan
 
 
 class ExchangeState:

@@ -1,0 +1,114 @@
+"""The FP64 reference gradient (oracle/adjoint.py) against central finite
+differences of the FP64 oracle forward path (h = 1e-4), on a small room
+with one BRDF per patch: what pins the reference the GPU adjoint is
+checked against."""
+import numpy as np
+import pytest
+
+import oracle as orc
+from oracle import adjoint as oadj
+from conftest import make_oracle
+from sparrowpy_amd import brdf as B
+from sparrowpy_amd import samplings, scenes, testing
+from sparrowpy_amd.coordinates import FrequencyData
+
+FREQS = np.array([500., 1000.])
+
+
+def small_room(size=(2., 1.5, 1.), ps=.5):
+    walls = scenes._split_walls_into_patches(
+        testing.shoebox_room_stub(*size), ps)
+    dirs = samplings.hemisphere(samplings.sph_gaussian(sh_order=3))
+    return walls, dirs
+
+
+def spec_info(dirs):
+    br = B.create_from_scattering(dirs, dirs,
+                                  FrequencyData(np.zeros((1, 1)), [1000.]),
+                                  FrequencyData(np.zeros((1, 1)), [1000.]))
+    z = br.freq[..., 0]
+    m = np.argmax(z, axis=1)
+    return z[np.arange(z.shape[0]), m], m
+
+
+def oracle_scene(walls, dirs, alpha, scat, air=(2e-3, 5e-3)):
+    r = make_oracle(walls, .5)  # one patch per (already split) wall
+    for n in range(len(walls)):
+        br = B.create_from_scattering(
+            dirs, dirs, FrequencyData(scat[n][None, :], FREQS),
+            FrequencyData(alpha[n][None, :], FREQS))
+        r.set_wall_brdf([n], br.freq, dirs.cartesian, dirs.cartesian, FREQS)
+    r.set_air_attenuation(np.asarray(air), FREQS)
+    r.bake_geometry()
+    return r
+
+
+def run(r, source, receivers, T, c, dt, K):
+    r.init_source_energy(source)
+    r.calculate_energy_exchange(c, dt, T * dt, K)
+    return r.collect_energy_receiver_mono(receivers)
+
+
+def loss_and_grad(H, target, db=60.0):
+    peak = target.max(axis=-1, keepdims=True)
+    mask = (target >= peak * 10 ** (-db / 10)) & (target > 0) & (H > 0)
+    d = np.where(mask, np.log10(np.where(mask, H, 1.0))
+                 - np.log10(np.where(mask, target, 1.0)), 0.0)
+    g = np.where(mask, 2 * d / (np.where(mask, H, 1.0) * np.log(10)), 0.0)
+    return float((d * d).sum()), g
+
+
+def _setup(seed=0):
+    walls, dirs = small_room()
+    rng = np.random.default_rng(seed)
+    n = len(walls)
+    true_a = rng.uniform(.05, .4, (n, 2))
+    true_s = rng.uniform(.1, .9, (n, 2))
+    alpha = np.clip(true_a + rng.normal(0, .05, (n, 2)), .02, .6)
+    scat = np.clip(true_s + rng.normal(0, .1, (n, 2)), .05, .95)
+    source = np.array([.6, .5, .4])
+    receivers = np.array([[1.4, 1.0, .6], [.9, .4, .7]])
+    T, c, dt, K = 120, 343., 1 / 2000, 3
+    return walls, dirs, true_a, true_s, alpha, scat, source, receivers, \
+        (T, c, dt, K)
+
+
+def test_oracle_adjoint_matches_finite_differences():
+    walls, dirs, ta, ts, alpha, scat, src, rcv, (T, c, dt, K) = _setup()
+    target = run(oracle_scene(walls, dirs, ta, ts), src, rcv, T, c, dt, K)
+    r = oracle_scene(walls, dirs, alpha, scat)
+    H = run(r, src, rcv, T, c, dt, K)
+    L, gH = loss_and_grad(H, target)
+    assert L > 0
+    gt = oadj.gradient(r, rcv, gH, K)
+    sv, m = spec_info(dirs)
+    ga, gs = oadj.scattering_gradient(gt, alpha, scat, sv, m)
+    rng = np.random.default_rng(1)
+    h = 1e-4
+    gmax = max(np.abs(ga).max(), np.abs(gs).max())
+    for kind in ("alpha", "s"):
+        for n, b in zip(rng.integers(0, len(walls), 8),
+                        rng.integers(0, 2, 8)):
+            vals = []
+            for sign in (1, -1):
+                a2, s2 = alpha.copy(), scat.copy()
+                (a2 if kind == "alpha" else s2)[n, b] += sign * h
+                Hs = run(oracle_scene(walls, dirs, a2, s2), src, rcv, T, c,
+                         dt, K)
+                vals.append(loss_and_grad(Hs, target)[0])
+            fd = (vals[0] - vals[1]) / (2 * h)
+            an = (ga if kind == "alpha" else gs)[n, b]
+            assert abs(fd - an) <= 1e-6 * gmax + 1e-5 * abs(fd), \
+                (kind, n, b, fd, an)
+
+
+def test_exchange_orders_sum_to_oracle_exchange():
+    """The order-by-order restatement sums to the C oracle's histogram."""
+    walls, dirs, ta, ts, alpha, scat, src, rcv, (T, c, dt, K) = _setup()
+    r = oracle_scene(walls, dirs, alpha, scat)
+    r.init_source_energy(src)
+    r.calculate_energy_exchange(c, dt, T * dt, K)
+    E = oadj.exchange_orders(T, r.energy_init_source,
+                             r.distance_patches_to_source, r.pair_distances(),
+                             r.fft, r.p2o, c, dt, K, r.visible_patches)
+    np.testing.assert_allclose(sum(E), r.etc, rtol=1e-12, atol=1e-300)
