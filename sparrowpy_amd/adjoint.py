@@ -88,7 +88,8 @@ class ExchangeAdjoint:
         self.brdf_index = ops.as_dev(b["brdf_index"], torch.int64)
         self.in_dirs = ops.as_dev(b["sources"], torch.float64)
         # receivers
-        self.rpts, _, self.rfactor, self.ridx = r._receiver_terms(receivers)
+        pts, _, self.rfactor, self.ridx = r._receiver_terms(receivers)
+        self.rpts = ops.as_dev(pts, torch.float64)
         self.air = ops.as_dev(b["air"] if b["air"] is not None
                               else np.zeros(self.B), torch.float64)
         self.R = self.rpts.shape[0]
@@ -250,9 +251,9 @@ def specular_factors(directions):
 def scattering_tables(alpha, scattering, spec_value):
     """Diffuse and specular tables of ``create_from_scattering`` BRDFs:
     for BRDF n with absorption alpha[n, b] and scattering s[n, b] and
-    incoming direction s_in, row k = n * S + s_in holds
-    A = (1 - alpha) s / pi and C = (1 - alpha)(1 - s) spec_value[s_in]
-    (times pi as the radiosity class stores them: A * pi, C * pi).
+    incoming direction s_in, row k = n * S + s_in holds, as the radiosity
+    class stores BRDFs (times pi), A = (1 - alpha) s and
+    C = (1 - alpha)(1 - s) pi spec_value[s_in].
     Differentiable in alpha and scattering ([NBRDF, B] tensors)."""
     sv = torch.as_tensor(np.asarray(spec_value), dtype=alpha.dtype,
                          device=alpha.device)

@@ -160,7 +160,7 @@ __global__ __launch_bounds__(256) void adjoint_ds_kernel(
         const bool in = qq + c < q1;
         const int64_t qc = in ? qq + c : q1 - 1;
         const int64_t j = tj[qc], m = tm[qc];
-        const int n = tlag[qc] & kLagMask;
+        const int n = min(tlag[qc] & kLagMask, T);  // >= T: reads 0
         const __amdgpu_buffer_rsrc_t rx = make_rsrc(
             musum + (j * B + b) * (int64_t)T, in ? row_bytes : 0u);
         const __amdgpu_buffer_rsrc_t ry = make_rsrc(

@@ -1,0 +1,134 @@
+"""GPU adjoint (sparrowpy_amd.adjoint) against the FP64 reference gradient
+(oracle/adjoint.py, itself pinned by finite differences) and against
+central finite differences of the FP64 oracle forward, on a small room
+with one BRDF per patch.  Tolerance: 1e-4 of the largest gradient
+component (the GPU histograms are float32)."""
+import numpy as np
+import pytest
+
+from oracle import adjoint as oadj
+from test_oracle_adjoint import (FREQS, _setup, loss_and_grad, oracle_scene,
+                                 run, spec_info)
+from sparrowpy_amd import brdf as B
+from sparrowpy_amd.coordinates import Coordinates, FrequencyData
+
+pytestmark = pytest.mark.gpu
+
+AIR = np.array([2e-3, 5e-3])
+
+
+@pytest.fixture(scope="module")
+def torch():
+    import torch as _torch
+    if not _torch.cuda.is_available():
+        pytest.skip("no GPU")
+    return _torch
+
+
+def gpu_scene(walls, dirs, alpha, scat):
+    from sparrowpy_amd import DirectionalRadiosityFast
+    r = DirectionalRadiosityFast.from_polygon(walls, .5)
+    for n in range(len(walls)):
+        br = B.create_from_scattering(
+            dirs, dirs, FrequencyData(scat[n][None, :], FREQS),
+            FrequencyData(alpha[n][None, :], FREQS))
+        r.set_wall_brdf([n], br, dirs, dirs)
+    r.set_air_attenuation(FrequencyData(AIR, FREQS))
+    r.bake_geometry()
+    return r
+
+
+def _adjoint(torch, walls, dirs, alpha, scat, src, rcv, T, c, dt, K, every):
+    from sparrowpy_amd import adjoint
+    r = gpu_scene(walls, dirs, alpha, scat)
+    adj = adjoint.ExchangeAdjoint(r, src, Coordinates(*rcv.T), c, dt,
+                                  T * dt, K, checkpoint_every=every)
+    return r, adj
+
+
+def test_forward_matches_class_path(torch):
+    """Histograms from the tables equal the drop-in class's (float32)."""
+    from sparrowpy_amd import adjoint
+    walls, dirs, ta, ts, alpha, scat, src, rcv, (T, c, dt, K) = _setup()
+    r, adj = _adjoint(torch, walls, dirs, alpha, scat, src, rcv, T, c, dt, K,
+                      2)
+    sv, _ = spec_info(dirs)
+    a = torch.tensor(alpha, dtype=torch.float64, device="cuda")
+    s = torch.tensor(scat, dtype=torch.float64, device="cuda")
+    A, C = adjoint.scattering_tables(a, s, sv)
+    np.testing.assert_allclose(A.cpu().numpy(), adj.A0.cpu().numpy(),
+                               rtol=1e-14)
+    np.testing.assert_allclose(C.cpu().numpy(), adj.C0.cpu().numpy(),
+                               rtol=1e-12)
+    H = adj.forward(A, C).cpu().numpy()
+    r.init_source_energy(Coordinates(*src))
+    r.calculate_energy_exchange(c, dt, T * dt, K)
+    ref = r.collect_energy_receiver_mono(Coordinates(*rcv.T)).time
+    from conftest import rel_err_within_db
+    assert rel_err_within_db(H, ref) <= 1e-5
+
+
+@pytest.mark.parametrize("every", [1, 2, 5])
+def test_gradient_matches_fp64_reference(torch, every):
+    from sparrowpy_amd import adjoint
+    walls, dirs, ta, ts, alpha, scat, src, rcv, (T, c, dt, K) = _setup()
+    target = run(oracle_scene(walls, dirs, ta, ts), src, rcv, T, c, dt, K)
+    # FP64 reference gradient
+    ro = oracle_scene(walls, dirs, alpha, scat)
+    Ho = run(ro, src, rcv, T, c, dt, K)
+    L, gH = loss_and_grad(Ho, target)
+    sv, m = spec_info(dirs)
+    ga_ref, gs_ref = oadj.scattering_gradient(
+        oadj.gradient(ro, rcv, gH, K), alpha, scat, sv, m)
+    # GPU adjoint through torch autograd
+    _, adj = _adjoint(torch, walls, dirs, alpha, scat, src, rcv, T, c, dt, K,
+                      every)
+    a = torch.tensor(alpha, dtype=torch.float64, device="cuda",
+                     requires_grad=True)
+    s = torch.tensor(scat, dtype=torch.float64, device="cuda",
+                     requires_grad=True)
+    A, C = adjoint.scattering_tables(a, s, sv)
+    H = adjoint.receiver_histograms(adj, A, C)
+    loss = adjoint.log_energy_loss(H, target)
+    loss.backward()
+    ga, gs = a.grad.cpu().numpy(), s.grad.cpu().numpy()
+    gmax = max(np.abs(ga_ref).max(), np.abs(gs_ref).max())
+    err = max(np.abs(ga - ga_ref).max(), np.abs(gs - gs_ref).max()) / gmax
+    print(f"[adjoint every={every}] loss gpu {float(loss):.6e} fp64 {L:.6e}"
+          f" max grad err / max grad {err:.2e}")
+    assert abs(float(loss) - L) <= 1e-5 * L
+    assert err <= 1e-4
+
+
+def test_gradient_matches_finite_differences(torch):
+    """GPU gradient against central differences (h = 1e-4) of the FP64
+    oracle loss on 10 random parameters."""
+    from sparrowpy_amd import adjoint
+    walls, dirs, ta, ts, alpha, scat, src, rcv, (T, c, dt, K) = _setup()
+    target = run(oracle_scene(walls, dirs, ta, ts), src, rcv, T, c, dt, K)
+    sv, _ = spec_info(dirs)
+    _, adj = _adjoint(torch, walls, dirs, alpha, scat, src, rcv, T, c, dt, K,
+                      2)
+    a = torch.tensor(alpha, dtype=torch.float64, device="cuda",
+                     requires_grad=True)
+    s = torch.tensor(scat, dtype=torch.float64, device="cuda",
+                     requires_grad=True)
+    A, C = adjoint.scattering_tables(a, s, sv)
+    adjoint.log_energy_loss(adjoint.receiver_histograms(adj, A, C),
+                            target).backward()
+    ga, gs = a.grad.cpu().numpy(), s.grad.cpu().numpy()
+    gmax = max(np.abs(ga).max(), np.abs(gs).max())
+    rng = np.random.default_rng(7)
+    h = 1e-4
+    for kind, n, b in zip(rng.choice(["alpha", "s"], 10),
+                          rng.integers(0, len(walls), 10),
+                          rng.integers(0, 2, 10)):
+        vals = []
+        for sign in (1, -1):
+            a2, s2 = alpha.copy(), scat.copy()
+            (a2 if kind == "alpha" else s2)[n, b] += sign * h
+            Hs = run(oracle_scene(walls, dirs, a2, s2), src, rcv, T, c, dt, K)
+            vals.append(loss_and_grad(Hs, target)[0])
+        fd = (vals[0] - vals[1]) / (2 * h)
+        g = (ga if kind == "alpha" else gs)[n, b]
+        assert abs(g - fd) <= 1e-4 * gmax, (kind, n, b, g, fd)
