@@ -146,6 +146,60 @@ def measured_traffic(kernel, config):
     return None
 
 
+def adjoint_timing(log):
+    """Forward (with order checkpoints) and adjoint time of the receiver
+    histograms' gradient with respect to the BRDF tables, on the 10x8x3 m
+    directional shoebox (1,072 patches, one BRDF per patch, 64 directions,
+    6 bands, 6,000 bins, order 30, checkpoints every 5 orders, 4
+    receivers, log-energy loss against perturbed tables)."""
+    import torch
+    from sparrowpy_amd import DirectionalRadiosityFast, adjoint
+    from sparrowpy_amd.coordinates import Coordinates, FrequencyData
+    sc = _scene("shoebox")
+    r = DirectionalRadiosityFast.from_polygon(sc.walls, sc.patch_size)
+    for idx, br in sc.brdf_groups:
+        r.set_wall_brdf(idx, br, sc.directions, sc.directions)
+    r.set_air_attenuation(FrequencyData(sc.air_attenuation, sc.frequencies))
+    r.bake_geometry()
+    adj = adjoint.ExchangeAdjoint(
+        r, Coordinates(*sc.sources[0]), Coordinates(*sc.receivers.T),
+        sc.speed_of_sound, sc.time_resolution, sc.duration, sc.max_order,
+        checkpoint_every=5)
+    g = torch.Generator(device="cuda").manual_seed(0)
+    A0, C0 = adj.A0, adj.C0
+    target = adj.forward(A0 * (1 + 0.1 * torch.rand(A0.shape, generator=g,
+                                                      device="cuda",
+                                                      dtype=A0.dtype)),
+                         C0).to(torch.float64)
+    adj.release()
+
+    def once():
+        A = A0.clone().requires_grad_(True)
+        C = C0.clone().requires_grad_(True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        H = adjoint.receiver_histograms(adj, A, C)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        adjoint.log_energy_loss(H, target).backward()
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        return t1 - t0, t2 - t1, float(A.grad.abs().max())
+
+    once()  # warm-up
+    fwd, bwd, gmax = once()
+    log(f"adjoint: forward {fwd:.3f} s, backward {bwd:.3f} s")
+    out = {"workload": f"{sc.name}: P={adj.P}, D={adj.D}, B={adj.B}, "
+                       f"T={adj.T}, order {adj.K}, checkpoints every 5, "
+                       f"{adj.R} receivers",
+           "forward_s": fwd, "adjoint_s": bwd, "adjoint_over_forward":
+           bwd / fwd, "max_abs_grad": gmax,
+           "plan": "diffuse+specular", "directed_pairs": adj.plan.Vd}
+    del adj, r
+    torch.cuda.empty_cache()
+    return out
+
+
 def copy_calibration(n_bytes=1 << 32, reps=3):
     """Copy n_bytes device-to-device ``reps`` times (one kernel each)."""
     import torch
@@ -170,6 +224,8 @@ def main(argv=None):
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="target CPU time of the oracle baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true",
+                    help="skip the adjoint/forward timing")
     ap.add_argument("--plan", default="auto",
                     choices=["auto", "ds", "grouped", "generic"],
                     help="exchange plan (auto: grouped when sources share "
@@ -347,6 +403,10 @@ def main(argv=None):
             "profiles/exchange_street_canyon.md (rocprofv3 FETCH_SIZE x2 + "
             "WRITE_SIZE x1, per launch)")
         line["roofline"]["traffic_GB/s"] = traffic / dom_s / 1e9
+    if not args.no_adjoint:
+        del state, etc, hist, r
+        torch.cuda.empty_cache()
+        line["adjoint"] = adjoint_timing(log)
     if not args.no_cpu_baseline:
         log("cpu baseline sample")
         ups, nth, T_s, n_dest = cpu_baseline(plan, D, T, args.cpu_seconds)
