@@ -200,6 +200,81 @@ def adjoint_timing(log):
     return out
 
 
+def bench_batched(args, log):
+    """Many rooms and sources: the per-instance loop of the reference API
+    against the batched, graph-captured exchange on the same baked scenes.
+    Throughput = exchange updates of all instances / wall time of the
+    exchange and receiver collection (bake excluded, reported)."""
+    import torch
+    from sparrowpy_amd import DirectionalRadiosityFast, scenes
+    from sparrowpy_amd.batch import BatchedExchange, BatchItem, \
+        run_per_instance
+    from sparrowpy_amd.coordinates import FrequencyData
+    scs = scenes.batched_shoeboxes(args.rooms)
+    t0 = time.perf_counter()
+    items = []
+    for sc in scs:
+        r = DirectionalRadiosityFast.from_polygon(sc.walls, sc.patch_size)
+        for idx, br in sc.brdf_groups:
+            r.set_wall_brdf(idx, br, sc.directions, sc.directions)
+        r.set_air_attenuation(FrequencyData(sc.air_attenuation,
+                                            sc.frequencies))
+        r.bake_geometry()
+        items.append(BatchItem(r, sc.sources, sc.receivers))
+    torch.cuda.synchronize()
+    bake_s = time.perf_counter() - t0
+    n_inst = sum(len(sc.sources) for sc in scs)
+    log(f"baked {len(scs)} rooms ({n_inst} instances) in {bake_s:.1f} s")
+    c, dt, dur, K = (scs[0].speed_of_sound, scs[0].time_resolution,
+                     scs[0].duration, scs[0].max_order)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ref = run_per_instance(items, c, dt, dur, K)
+    torch.cuda.synchronize()
+    loop_s = time.perf_counter() - t0
+    log(f"per-instance loop {loop_s:.2f} s")
+    torch.cuda.empty_cache()
+    be = BatchedExchange(items, c, dt, dur, K)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    out = be.run()
+    torch.cuda.synchronize()
+    batched_s = time.perf_counter() - t0
+    log(f"batched {batched_s:.2f} s, classes {be.choice}")
+    err = 0.0
+    for key, v in ref.items():
+        a, b = out[key].double(), v.double()
+        peak = b.abs().amax(dim=-1, keepdim=True)
+        mask = b.abs() >= peak * 1e-6
+        if mask.any():
+            err = max(err, float(((a - b).abs() / b.abs())[mask].max()))
+    U = be.updates
+    return {
+        "metric": "patch-pair·bin updates/s, batched rooms",
+        "value": U / batched_s, "unit": "updates/s", "n_gpus": 1,
+        "steps": 1, "warmup": 0, "ms_per_step": batched_s * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 state, f64 accumulate",
+        "data": "synthetic (seeded shoeboxes)",
+        "config": {"workload": f"{len(scs)} shoeboxes [3,20]x[3,20]x"
+                               f"[2.5,6] m, patch [0.5,1.5] m, 1-4 sources,"
+                               f" 1-8 receivers, D 16/64, 1-8 bands, "
+                               f"2 kHz x 1 s, order {K}",
+                   "rooms": len(scs), "instances": n_inst,
+                   "patches": int(sum(it.radiosity.n_patches
+                                      for it in items)),
+                   "updates": U,
+                   "plan_per_class": {f"D{k[0]}xB{k[1]}": v
+                                      for k, v in be.choice.items()}},
+        "per_instance_loop": {"s": loop_s, "updates/s": U / loop_s},
+        "speedup_vs_loop": loop_s / batched_s,
+        "max_rel_err_vs_loop": err, "bake_s": bake_s,
+        "includes": "exchange, order-0 injection, receiver collection; "
+                    "batched also plan concatenation, autotuning and "
+                    "graph capture",
+    }
+
+
 def copy_calibration(n_bytes=1 << 32, reps=3):
     """Copy n_bytes device-to-device ``reps`` times (one kernel each)."""
     import torch
@@ -220,10 +295,12 @@ def main(argv=None):
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default="canyon",
-                    choices=["canyon", "shoebox", "small"])
+                    choices=["canyon", "shoebox", "small", "batched"])
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="target CPU time of the oracle baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--rooms", type=int, default=256,
+                    help="rooms of --config batched")
     ap.add_argument("--no-adjoint", action="store_true",
                     help="skip the adjoint/forward timing")
     ap.add_argument("--plan", default="auto",
@@ -246,6 +323,11 @@ def main(argv=None):
     from sparrowpy_amd import DirectionalRadiosityFast, ops
     from sparrowpy_amd.coordinates import Coordinates, FrequencyData
 
+    if args.config == "batched":
+        def blog(msg):
+            print(f"[bench] {msg}", file=sys.stderr, flush=True)
+        print(json.dumps(bench_batched(args, blog)), flush=True)
+        return 0
     sc = _scene(args.config)
     torch.cuda.init()
     dev = torch.device("cuda", 0)

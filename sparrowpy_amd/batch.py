@@ -1,0 +1,279 @@
+"""Many scenes and sources in one exchange.
+
+The reference runs one source of one scene per ``calculate_energy_exchange``
+call (``RadiosityFast.py:524-568``; a multi-source study loops in Python,
+``tests/test_multisource.py:82-132``).  Here every (scene, source) pair is
+an *instance*: instances whose scenes share the number of directions and
+bands and the plan kind form a class, and a class is run as one exchange
+over the concatenation of its instances' patches -- a block-diagonal plan
+(each instance's rows only reference its own patches), so one launch pair
+per order covers the whole class and results are bit-for-bit those of the
+per-instance runs (same entries, same per-row order).  The K orders of a
+class are captured once into a HIP graph and replayed; per class the plan
+kind (grouped or diffuse + specular) is picked by timing one order of each
+candidate on the class's largest scene (``autotune``).  Classes larger than the memory budget are
+split into chunks.
+"""
+from __future__ import annotations
+
+import dataclasses
+import time
+
+import numpy as np
+import torch
+
+from . import ops
+
+__all__ = ["BatchItem", "BatchedExchange", "run_per_instance"]
+
+
+@dataclasses.dataclass
+class BatchItem:
+    """One baked scene (a DirectionalRadiosityFast with BRDFs and air set),
+    its sources [n, 3] and receivers (Coordinates or [R, 3])."""
+
+    radiosity: object
+    sources: np.ndarray
+    receivers: object
+
+
+def _cat(ts, dim=0):
+    return torch.cat([t.reshape(t.shape) for t in ts], dim=dim).contiguous()
+
+
+def _concat_plans(plans, offsets):
+    """Block-diagonal concatenation of per-instance plans (patch offsets)."""
+    p0 = plans[0]
+    D = p0.D
+    dev = p0.row_ptr.device
+    vd = [p.Vd for p in plans]
+    voff = np.concatenate([[0], np.cumsum(vd)]).astype(np.int64)
+    rows = [p.row_ptr[:-1] + int(voff[n]) for n, p in enumerate(plans)]
+    rows.append(torch.tensor([voff[-1]], dtype=torch.int64, device=dev))
+    row_ptr = torch.cat(rows)
+    P = int(sum(p.P for p in plans))
+    lag = _cat([p.lag for p in plans])
+    if isinstance(p0, ops.GroupedPlan):
+        gs = [p.n_groups for p in plans]
+        goff = np.concatenate([[0], np.cumsum(gs)]).astype(np.int64)
+        ks = [p.rtab.shape[0] for p in plans]
+        koff = np.concatenate([[0], np.cumsum(ks)]).astype(np.int64)
+        grp = [p.grp_ptr[:-1] + int(goff[n]) for n, p in enumerate(plans)]
+        grp.append(torch.tensor([goff[-1]], dtype=torch.int64, device=dev))
+        starts = [p.grp_start[:-1] + int(voff[n]) for n, p in
+                  enumerate(plans)]
+        starts.append(torch.tensor([voff[-1]], dtype=torch.int64,
+                                   device=dev))
+        out = ops.GroupedPlan(
+            P, D, p0.B, row_ptr, torch.cat(grp), torch.cat(starts),
+            _cat([p.grp_key + int(koff[n]) for n, p in enumerate(plans)]),
+            _cat([p.rowid + int(offsets[n]) * D for n, p in
+                  enumerate(plans)]),
+            lag, _cat([p.scal for p in plans], 1),
+            _cat([p.rtab for p in plans]),
+            _cat([p.dscal for p in plans], 1) if p0.diffuse_specular
+            else None)
+        return out
+    return ops.ExchangePlan(
+        P, D, p0.B, row_ptr,
+        _cat([p.src + int(offsets[n]) for n, p in enumerate(plans)]),
+        lag, _cat([p.dir for p in plans]), _cat([p.coef for p in plans]))
+
+
+def _kind(plan):
+    if isinstance(plan, ops.GroupedPlan):
+        return "ds" if plan.diffuse_specular else "grouped"
+    return "generic"
+
+
+class BatchedExchange:
+    """Exchange and receiver collection of many instances at once.
+
+    ``items``: list of :class:`BatchItem`; all scenes share the histogram
+    settings (speed of sound, bin width, duration, order).
+    """
+
+    def __init__(self, items, speed_of_sound, time_resolution, duration,
+                 max_order, autotune=True, memory_budget=None, graphs=True):
+        self.items = list(items)
+        self.c, self.dt = float(speed_of_sound), float(time_resolution)
+        self.T = int(duration / time_resolution)
+        self.K = int(max_order)
+        self.graphs = graphs
+        dev = ops.device()
+        if memory_budget is None:
+            memory_budget = torch.cuda.get_device_properties(
+                dev).total_memory // 2
+        self.budget = int(memory_budget)
+        self.autotune = autotune
+        self.choice = {}          # class key -> plan kind
+        self._receivers = {}
+        for x, it in enumerate(self.items):
+            r = it.radiosity
+            pts, _, factor, idx = r._receiver_terms(it.receivers)
+            g = r._geometry()
+            self._receivers[x] = (ops.as_dev(pts, torch.float64), factor,
+                                  idx, g["centers"],
+                                  ops.as_dev(r._air_attenuation,
+                                             torch.float64))
+
+    # -- planning ------------------------------------------------------------
+    def _plan(self, x, kind):
+        r = self.items[x].radiosity
+        try:
+            return r._exchange_plan(self.c, self.dt, kind)
+        except ValueError:
+            return None
+
+    def _classes(self):
+        cls = {}
+        for x, it in enumerate(self.items):
+            b = it.radiosity._bake
+            cls.setdefault((b["D"], b["n_bins"]), []).append(x)
+        return cls
+
+    def _instance_bytes(self, x, plan):
+        r = self.items[x].radiosity
+        P = r.n_patches
+        D, B = r._bake["D"], r._bake["n_bins"]
+        G = plan.n_groups if isinstance(plan, ops.GroupedPlan) else 0
+        return 4 * self.T * B * (4 * P * D + G)
+
+    def _chunks(self, xs, plans):
+        """(scene, source) instances of a class, packed into chunks."""
+        chunks, cur, used = [], [], 0
+        for x in xs:
+            nb = self._instance_bytes(x, plans[x])
+            for s in range(len(self.items[x].sources)):
+                if cur and used + nb > self.budget:
+                    chunks.append(cur)
+                    cur, used = [], 0
+                cur.append((x, s))
+                used += nb
+        if cur:
+            chunks.append(cur)
+        return chunks
+
+    def _pick_kind(self, key, xs):
+        """Plan kind of a class: the faster of the grouped and diffuse +
+        specular plans (when every scene of the class supports it), timed
+        on one order of the class's largest scene; generic otherwise."""
+        kinds = [k for k in ("grouped", "ds")
+                 if all(self._plan(x, k) is not None for x in xs)]
+        if not kinds:
+            return "generic"
+        if len(kinds) == 1 or not self.autotune:
+            return kinds[0]
+        best, best_t = None, None
+        x_big = max(xs, key=lambda x: self.items[x].radiosity.n_patches)
+        for k in kinds:
+            plans = {x_big: self._plan(x_big, k)}
+            run = self._build([(x_big, 0)], plans)
+            st = run["state"]
+            st.step(run["plan"])             # warm-up
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            st.step(run["plan"])
+            torch.cuda.synchronize()
+            t = time.perf_counter() - t0
+            del run, st
+            if best_t is None or t < best_t:
+                best, best_t = k, t
+        return best
+
+    def _build(self, chunk, plans):
+        xs = [x for x, _ in chunk]
+        offsets = np.concatenate(
+            [[0], np.cumsum([self.items[x].radiosity.n_patches
+                             for x in xs])]).astype(np.int64)
+        plan = _concat_plans([plans[x] for x in xs], offsets[:-1])
+        e0d, d0 = [], []
+        for x, s in chunk:
+            r = self.items[x].radiosity
+            r.init_source_energy(self.items[x].sources[s])
+            e0d.append(r._dev["energy_init_source"])
+            d0.append(r._dev["distance_patches_to_source"])
+        e0d, d0 = _cat(e0d), _cat(d0)
+        P, D, B = e0d.shape
+        st = ops.ExchangeState(P, D, B, self.T)
+        st.init(e0d, d0, self.c, self.dt)
+        if isinstance(plan, ops.GroupedPlan):
+            st._gsum(plan)
+        return dict(plan=plan, state=st, e0d=e0d, d0=d0, offsets=offsets,
+                    chunk=chunk)
+
+    # -- running ---------------------------------------------------------------
+    def _exchange(self, run):
+        st, plan = run["state"], run["plan"]
+        if not self.graphs or self.K == 0:
+            for _ in range(self.K):
+                st.step(plan)
+            return
+        a, b = st.prev, st.cur
+        graph = torch.cuda.CUDAGraph()
+        side = torch.cuda.Stream()
+        side.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(side):
+            with torch.cuda.graph(graph, stream=side):
+                for _ in range(self.K):
+                    st.step(plan)
+        torch.cuda.current_stream().wait_stream(side)
+        # capture recorded the launches without running them: reset the
+        # state to order 0 and replay
+        st.prev, st.cur = a, b
+        st.init(run["e0d"], run["d0"], self.c, self.dt)
+        graph.replay()
+        if self.K % 2:
+            st.prev, st.cur = b, a
+        run["graph"] = graph
+
+    def run(self):
+        """{(scene, source): receiver histograms float32 [R, B, T]}."""
+        out = {}
+        self.updates = 0
+        for key, xs in self._classes().items():
+            kind = self._pick_kind(key, xs)
+            self.choice[key] = kind
+            plans = {x: self._plan(x, kind) for x in xs}
+            for chunk in self._chunks(xs, plans):
+                run = self._build(chunk, plans)
+                self._exchange(run)
+                st = run["state"]
+                st.check()
+                total = st.finalize(out=st.hi)
+                off = run["offsets"]
+                for n, (x, s) in enumerate(chunk):
+                    pts, factor, idx, centers, air = self._receivers[x]
+                    slab = total[int(off[n]):int(off[n + 1])]
+                    out[(x, s)] = ops.collect(slab, pts, centers, idx,
+                                              factor, air, self.c, self.dt)
+                plan = run["plan"]
+                lag = plan.lag.to(torch.int64) & (
+                    ops.GroupedPlan.LAST_IN_GROUP - 1)
+                self.updates += int(
+                    (st.shape[2] * torch.clamp(self.T - lag, min=0)).sum()
+                    .item()) * self.K
+                del run, st, total
+        return out
+
+
+def run_per_instance(items, speed_of_sound, time_resolution, duration,
+                     max_order):
+    """The reference's loop: one ``init_source_energy`` /
+    ``calculate_energy_exchange`` / ``collect_energy_receiver_mono`` per
+    (scene, source).  {(scene, source): float32 [R, B, T] device tensor}."""
+    out = {}
+    for x, it in enumerate(items):
+        r = it.radiosity
+        pts, _, factor, idx = r._receiver_terms(it.receivers)
+        g = r._geometry()
+        air = ops.as_dev(r._air_attenuation, torch.float64)
+        for s, src in enumerate(it.sources):
+            r.init_source_energy(src)
+            r.calculate_energy_exchange(speed_of_sound, time_resolution,
+                                        duration, max_order,
+                                        recalculate=True)
+            out[(x, s)] = ops.collect(r._etc, pts, g["centers"], idx, factor,
+                                      air, speed_of_sound, time_resolution)
+            r._etc = None
+    return out

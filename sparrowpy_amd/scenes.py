@@ -19,7 +19,7 @@ from .samplings import hemisphere, sph_gaussian
 from .testing import shoebox_room_stub
 
 __all__ = ["Scene", "small_shoebox", "directional_shoebox", "street_canyon",
-           "OCTAVE_BANDS", "air_attenuation_octaves"]
+           "batched_shoeboxes", "OCTAVE_BANDS", "air_attenuation_octaves"]
 
 OCTAVE_BANDS = np.array([63., 125., 250., 500., 1000., 2000., 4000., 8000.])
 
@@ -211,3 +211,42 @@ def street_canyon(seed=3, length=100., width=20., height=15.,
         directions=dirs, air_attenuation=air_attenuation_octaves(freqs),
         sources=src, receivers=recv, speed_of_sound=343.,
         time_resolution=1 / rate, duration=duration, max_order=max_order)
+
+
+def batched_shoeboxes(n_scenes=256, seed=4, rate=2000., duration=1.0,
+                      max_order=20):
+    """Many small rooms for batched runs: shoeboxes with dimensions uniform
+    in [3,20] x [3,20] x [2.5,6] m and patch size uniform in [0.5,1.5] m
+    (28 to ~3,000 patches), 1-4 sources and 1-8 receivers at least 0.5 m
+    from the walls, 16 or 64 BRDF directions, 1-8 octave bands, one seeded
+    material per room (alpha in [0.05,0.3], s in [0.1,0.9] per band),
+    2 kHz bins over 1 s, order 20."""
+    rng = np.random.default_rng(seed)
+    dirs = {16: hemisphere(sph_gaussian(sh_order=3)),
+            64: hemisphere(sph_gaussian(sh_order=7))}
+    out = []
+    for x in range(n_scenes):
+        X, Y = rng.uniform(3, 20, 2)
+        Z = rng.uniform(2.5, 6)
+        ps = float(rng.uniform(0.5, 1.5))
+        n_src, n_rcv = int(rng.integers(1, 5)), int(rng.integers(1, 9))
+        D = int(rng.choice([16, 64]))
+        n_bands = int(rng.integers(1, 9))
+        freqs = OCTAVE_BANDS[:n_bands]
+
+        def inside(n):
+            return np.stack([rng.uniform(.5, X - .5, n),
+                             rng.uniform(.5, Y - .5, n),
+                             rng.uniform(.5, Z - .5, n)], axis=-1)
+        walls = shoebox_room_stub(X, Y, Z)
+        brdf = _brdf(dirs[D], rng.uniform(0.1, 0.9, n_bands),
+                     rng.uniform(0.05, 0.3, n_bands), freqs)
+        out.append(Scene(
+            name=f"room{x}-{X:.1f}x{Y:.1f}x{Z:.1f}-p{ps:.2f}", walls=walls,
+            patch_size=ps, frequencies=freqs,
+            brdf_groups=[(np.arange(len(walls)), brdf)], directions=dirs[D],
+            air_attenuation=air_attenuation_octaves(freqs),
+            sources=inside(n_src), receivers=inside(n_rcv),
+            speed_of_sound=343., time_resolution=1 / rate, duration=duration,
+            max_order=max_order))
+    return out
