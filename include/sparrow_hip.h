@@ -411,8 +411,9 @@ int spr_plan_ds_transpose_rows(int64_t P, int64_t D, const int64_t *row_ptr,
  * te[q] (forward entry index), tj[q] (destination), tlag[q] (delay, bit 30
  * on the last entry of a direction group), tm[q] (specular direction of
  * the entry, 0 if none), tgptr[P*D+1] (int64: first entry of source i with
- * direction >= d at i*D + d).  cursor[P] (int64) and tkey[Vd] (int32) are
- * workspace; key[Vd] is the plan's entry key (specular direction or D). */
+ * direction >= d at i*D + d), tkey[Vd] (int32: direction << 25 |
+ * destination).  cursor[P] (int64) is workspace; key[Vd] is the plan's
+ * entry key (specular direction or D). */
 int spr_plan_ds_transpose_fill(int64_t P, int64_t D, int64_t Vd,
                                int64_t max_trow, const int64_t *row_ptr,
                                const int32_t *rowid, const int32_t *lag,
@@ -439,15 +440,17 @@ int spr_collect_adjoint(int64_t NR, int64_t P, int64_t D, int64_t B,
 int spr_direction_sum(int64_t P, int64_t D, int64_t B, int64_t T,
                       const float *mu, float *musum, void *stream);
 
-/* One backward order: muout = lam + A^T munext (the transposed exchange),
- * with musum = spr_direction_sum(munext) and ek the forward order-k
- * histogram; adds each transposed entry's table-gradient dot products to
- * ga[2,B,Vd] (float64): ga[0,b,q] += sum_t musum[j,b,t+n] ek[i,d,b,t],
- * ga[1,b,q] += sum_t munext[j,m,b,t+n] ek[i,d,b,t].  muout must not alias
- * an input.  T < 2^29. */
+/* One backward order: muout = lam + A^T munext (the transposed exchange;
+ * muout is first set to lam, then each direction row with entries gets its
+ * sum added), with musum = spr_direction_sum(munext) and ek the forward
+ * order-k histogram; adds each transposed entry's table-gradient dot
+ * products to ga[2,B,Vd] (float64): ga[0,b,q] += sum_t musum[j,b,t+n]
+ * ek[i,d,b,t], ga[1,b,q] += sum_t munext[j,m,b,t+n] ek[i,d,b,t].  tkey is
+ * spr_plan_ds_transpose_fill's tkey (direction << 25 | destination), kept
+ * after the fill.  muout must not alias an input.  T < 2^29, P < 2^25. */
 int spr_exchange_adjoint_ds(int64_t P, int64_t D, int64_t B, int64_t T,
                             int64_t Vd, const int64_t *tgptr,
-                            const int32_t *tj, const int32_t *tlag,
+                            const int32_t *tkey, const int32_t *tlag,
                             const int32_t *tm, const double *twa,
                             const double *twc, const float *lam,
                             const float *musum, const float *munext,

@@ -104,13 +104,13 @@ class ExchangeAdjoint:
         cnt = self.trow_ptr[1:] - self.trow_ptr[:-1]
         max_trow = int(cnt.max().item()) if P else 0
         cursor = torch.empty(P, dtype=torch.int64, device=dev)
-        self.te, tkey = _i32(Vd, dev), _i32(Vd, dev)
+        self.te, self.tkey = _i32(Vd, dev), _i32(Vd, dev)
         self.tj, self.tlag, self.tm = _i32(Vd, dev), _i32(Vd, dev), \
             _i32(Vd, dev)
         self.tgptr = torch.empty(P * D + 1, dtype=torch.int64, device=dev)
         call("spr_plan_ds_transpose_fill", P, D, Vd, max_trow,
              ptr(p.row_ptr), ptr(p.rowid), ptr(p.lag), ptr(adj["key"]),
-             ptr(self.trow_ptr), ptr(cursor), ptr(self.te), ptr(tkey),
+             ptr(self.trow_ptr), ptr(cursor), ptr(self.te), ptr(self.tkey),
              ptr(self.tj), ptr(self.tlag), ptr(self.tm), ptr(self.tgptr),
              stream_ptr())
 
@@ -196,7 +196,7 @@ class ExchangeAdjoint:
                 call("spr_direction_sum", P, D, B, T, ptr(mu), ptr(musum),
                      stream_ptr())
                 call("spr_exchange_adjoint_ds", P, D, B, T, Vd,
-                     ptr(self.tgptr), ptr(self.tj), ptr(self.tlag),
+                     ptr(self.tgptr), ptr(self.tkey), ptr(self.tlag),
                      ptr(self.tm), ptr(twa), ptr(twc), ptr(lam), ptr(musum),
                      ptr(mu), ptr(states[k - s]), ptr(mu_out), ptr(ga),
                      stream_ptr())

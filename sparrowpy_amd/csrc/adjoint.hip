@@ -101,123 +101,122 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
-// One backward order: mu_out = lambda + A^T mu_next, and the entries' table
-// gradients ga[0,b,q] += sum_t musum[j,b,t+n] ek[i,d',b,t],
-// ga[1,b,q] += sum_t mu_next[j,m,b,t+n] ek[i,d',b,t] (q: transposed entry).
-// A workgroup takes one (source i, band, block of 256*R bins), sources
-// fastest; it walks the directions d' of source i, and for each the
-// entries of the direction group in chunks of C, the next chunk's loads
-// in flight while the current one is summed.  A bin t + n beyond the
-// histogram is an out-of-range buffer offset and reads 0.  The per-entry
-// dot products are reduced over each wave and added (float64 atomics) by
-// lanes 0 .. 2C-1.  Requires T < 2^29.
+// One backward order: mu_out += A^T mu_next (mu_out holds lambda on entry),
+// and the entries' table gradients ga[0,b,q] += sum_t musum[j,b,t+n]
+// ek[i,d',b,t], ga[1,b,q] += sum_t mu_next[j,m,b,t+n] ek[i,d',b,t]
+// (q: transposed entry, d' its source direction).  A workgroup takes one
+// (source i, band, block of 256*R bins), sources fastest, and streams all
+// entries of source i -- sorted by direction, a flag on the last entry of
+// each direction -- in chunks of C, the next chunk's loads (mu-sum row,
+// specular mu row, the source's own order-k row) in flight while the
+// current one is summed.  At a direction's last entry the sum is added to
+// its mu_out row with float atomics (each element receives exactly one
+// add, so the result does not depend on their order).  A bin t + n beyond
+// the histogram is an out-of-range buffer offset and reads 0.  The
+// per-entry dot products are reduced over each wave and added (float64
+// atomics) by lanes 0 .. 2C-1.  Requires T < 2^29.
 template <int R, int C>
 __global__ __launch_bounds__(256) void adjoint_ds_kernel(
     int D, int B, int T, int n_tblk, int64_t nblocks, int64_t Vd,
-    const int64_t *__restrict__ tgptr, const int32_t *__restrict__ tj,
+    const int64_t *__restrict__ tgptr, const int32_t *__restrict__ tkey,
     const int32_t *__restrict__ tlag, const int32_t *__restrict__ tm,
     const double *__restrict__ twa, const double *__restrict__ twc,
-    const float *__restrict__ lam, const float *__restrict__ musum,
-    const float *__restrict__ munext, const float *__restrict__ ek,
-    float *__restrict__ muout, double *__restrict__ ga) {
+    const float *__restrict__ musum, const float *__restrict__ munext,
+    const float *__restrict__ ek, float *__restrict__ muout,
+    double *__restrict__ ga) {
   constexpr int BS = 256;
+  constexpr int32_t kDstMask = (1 << 25) - 1;
   int64_t i;
   int b, tb;
   dest_major_tile(xcd_tile(blockIdx.x, nblocks), nblocks / (B * n_tblk),
                   n_tblk, i, b, tb);
+  const int64_t q0 = tgptr[i * D], q1 = tgptr[i * D + D];
+  if (q1 == q0) return;
   const int tl = tb * (BS * R) + (int)threadIdx.x;
   const int lane = (int)threadIdx.x & 63;
   const uint32_t row_bytes = (uint32_t)T * 4u;
   const double *wa = twa + (int64_t)b * Vd, *wc = twc + (int64_t)b * Vd;
   double *gda = ga + (int64_t)b * Vd, *gdc = ga + ((int64_t)B + b) * Vd;
+  const int64_t srow = i * D;  // row (i, d') = srow + d'
 
-  for (int d = 0; d < D; ++d) {
-    const int64_t q0 = tgptr[i * D + d], q1 = tgptr[i * D + d + 1];
-    const int64_t roff = ((i * D + d) * B + b) * (int64_t)T;
-    const __amdgpu_buffer_rsrc_t rl = make_rsrc(lam + roff, row_bytes);
-    const __amdgpu_buffer_rsrc_t ro = make_rsrc(muout + roff, row_bytes);
-    float lv[R];
+  auto fetch = [&](int64_t qq, float (&x)[C][R], float (&y)[C][R],
+                   float (&e)[C][R]) {
 #pragma unroll
-    for (int r = 0; r < R; ++r)
-      lv[r] = buffer_load(rl, (uint32_t)(tl + BS * r) * 4u);
-    if (q1 == q0) {
+    for (int c = 0; c < C; ++c) {
+      const bool in = qq + c < q1;
+      const int64_t qc = in ? qq + c : q1 - 1;
+      const int32_t key = tkey[qc];
+      const int64_t j = key & kDstMask, d = key >> 25, m = tm[qc];
+      const int n = min(tlag[qc] & kLagMask, T);  // >= T: reads 0
+      const uint32_t sz = in ? row_bytes : 0u;
+      const __amdgpu_buffer_rsrc_t rx =
+          make_rsrc(musum + (j * B + b) * (int64_t)T, sz);
+      const __amdgpu_buffer_rsrc_t ry =
+          make_rsrc(munext + ((j * D + m) * B + b) * (int64_t)T, sz);
+      const __amdgpu_buffer_rsrc_t re =
+          make_rsrc(ek + ((srow + d) * B + b) * (int64_t)T, sz);
 #pragma unroll
-      for (int r = 0; r < R; ++r)
-        buffer_store(lv[r], ro, (uint32_t)(tl + BS * r) * 4u);
-      continue;
+      for (int r = 0; r < R; ++r) {
+        const int t = tl + BS * r;
+        x[c][r] = buffer_load(rx, (uint32_t)(t + n) * 4u);
+        y[c][r] = buffer_load(ry, (uint32_t)(t + n) * 4u);
+        e[c][r] = buffer_load(re, (uint32_t)t * 4u);
+      }
     }
-    const __amdgpu_buffer_rsrc_t re = make_rsrc(ek + roff, row_bytes);
-    float ev[R];
-    double acc[R];
+  };
+  double acc[R];
 #pragma unroll
-    for (int r = 0; r < R; ++r) {
-      ev[r] = buffer_load(re, (uint32_t)(tl + BS * r) * 4u);
-      acc[r] = 0.0;
-    }
-    auto fetch = [&](int64_t qq, float (&x)[C][R], float (&y)[C][R]) {
+  for (int r = 0; r < R; ++r) acc[r] = 0.0;
+  auto consume = [&](int64_t qq, const float (&x)[C][R],
+                     const float (&y)[C][R], const float (&e)[C][R]) {
+    double pa[C], pc[C];
 #pragma unroll
-      for (int c = 0; c < C; ++c) {
-        const bool in = qq + c < q1;
-        const int64_t qc = in ? qq + c : q1 - 1;
-        const int64_t j = tj[qc], m = tm[qc];
-        const int n = min(tlag[qc] & kLagMask, T);  // >= T: reads 0
-        const __amdgpu_buffer_rsrc_t rx = make_rsrc(
-            musum + (j * B + b) * (int64_t)T, in ? row_bytes : 0u);
-        const __amdgpu_buffer_rsrc_t ry = make_rsrc(
-            munext + ((j * D + m) * B + b) * (int64_t)T, in ? row_bytes : 0u);
+    for (int c = 0; c < C; ++c) {
+      pa[c] = pc[c] = 0.0;
+      if (qq + c < q1) {
+        const double a = wa[qq + c], s = wc[qq + c];
 #pragma unroll
         for (int r = 0; r < R; ++r) {
-          const uint32_t off = (uint32_t)(tl + BS * r + n) * 4u;
-          x[c][r] = buffer_load(rx, off);
-          y[c][r] = buffer_load(ry, off);
+          const double xd = (double)x[c][r], yd = (double)y[c][r];
+          const double ed = (double)e[c][r];
+          acc[r] = fma(a, xd, acc[r]);
+          acc[r] = fma(s, yd, acc[r]);
+          pa[c] = fma(ed, xd, pa[c]);
+          pc[c] = fma(ed, yd, pc[c]);
         }
-      }
-    };
-    auto consume = [&](int64_t qq, const float (&x)[C][R],
-                       const float (&y)[C][R]) {
-      double pa[C], pc[C];
-#pragma unroll
-      for (int c = 0; c < C; ++c) {
-        pa[c] = pc[c] = 0.0;
-        if (qq + c < q1) {
-          const double a = wa[qq + c], s = wc[qq + c];
+        if (tlag[qq + c] & kLastInGroup) {
+          const int64_t d = tkey[qq + c] >> 25;
+          float *o = muout + ((srow + d) * B + b) * (int64_t)T;
 #pragma unroll
           for (int r = 0; r < R; ++r) {
-            const double xd = (double)x[c][r], yd = (double)y[c][r];
-            acc[r] = fma(a, xd, acc[r]);
-            acc[r] = fma(s, yd, acc[r]);
-            pa[c] = fma((double)ev[r], xd, pa[c]);
-            pc[c] = fma((double)ev[r], yd, pc[c]);
+            const int t = tl + BS * r;
+            if (t < T) unsafeAtomicAdd(o + t, (float)acc[r]);
+            acc[r] = 0.0;
           }
         }
       }
-#pragma unroll
-      for (int c = 0; c < C; ++c) {
-        pa[c] = wave_sum(pa[c]);
-        pc[c] = wave_sum(pc[c]);
-      }
-      if (lane < 2 * C) {
-        const int c = lane % C;
-        double v = 0.0;
-#pragma unroll
-        for (int cc = 0; cc < C; ++cc)
-          if (cc == c) v = lane < C ? pa[cc] : pc[cc];
-        if (qq + c < q1)
-          unsafeAtomicAdd((lane < C ? gda : gdc) + qq + c, v);
-      }
-    };
-    float xa[C][R], ya[C][R], xb[C][R], yb[C][R];
-    fetch(q0, xa, ya);
-    for (int64_t q = q0; q < q1; q += 2 * C) {
-      fetch(q + C, xb, yb);
-      consume(q, xa, ya);
-      fetch(q + 2 * C, xa, ya);
-      consume(q + C, xb, yb);
     }
 #pragma unroll
-    for (int r = 0; r < R; ++r)
-      buffer_store((float)((double)lv[r] + acc[r]), ro,
-                   (uint32_t)(tl + BS * r) * 4u);
+    for (int c = 0; c < C; ++c) {
+      pa[c] = wave_sum(pa[c]);
+      pc[c] = wave_sum(pc[c]);
+    }
+    if (lane < 2 * C) {
+      const int c = lane % C;
+      double v = 0.0;
+#pragma unroll
+      for (int cc = 0; cc < C; ++cc)
+        if (cc == c) v = lane < C ? pa[cc] : pc[cc];
+      if (qq + c < q1) unsafeAtomicAdd((lane < C ? gda : gdc) + qq + c, v);
+    }
+  };
+  float xa[C][R], ya[C][R], ea[C][R], xb[C][R], yb[C][R], eb[C][R];
+  fetch(q0, xa, ya, ea);
+  for (int64_t q = q0; q < q1; q += 2 * C) {
+    fetch(q + C, xb, yb, eb);
+    consume(q, xa, ya, ea);
+    fetch(q + 2 * C, xa, ya, ea);
+    consume(q + C, xb, yb, eb);
   }
 }
 
@@ -340,15 +339,15 @@ extern "C" int spr_direction_sum(int64_t P, int64_t D, int64_t B, int64_t T,
 
 extern "C" int spr_exchange_adjoint_ds(
     int64_t P, int64_t D, int64_t B, int64_t T, int64_t Vd,
-    const int64_t *tgptr, const int32_t *tj, const int32_t *tlag,
+    const int64_t *tgptr, const int32_t *tkey, const int32_t *tlag,
     const int32_t *tm, const double *twa, const double *twc,
     const float *lam, const float *musum, const float *munext,
     const float *ek, float *muout, double *ga, void *stream) {
   SPR_REQUIRE(P > 0 && D > 0 && D <= 64 && B > 0 && T > 0 && Vd >= 0,
               "spr_exchange_adjoint_ds: bad sizes");
-  SPR_REQUIRE(T < ((int64_t)1 << 29),
-              "spr_exchange_adjoint_ds: histogram too long (T >= 2^29)");
-  SPR_REQUIRE(tgptr && tj && tlag && tm && twa && twc && lam && musum &&
+  SPR_REQUIRE(T < ((int64_t)1 << 29) && P < ((int64_t)1 << 25),
+              "spr_exchange_adjoint_ds: histogram or scene too large");
+  SPR_REQUIRE(tgptr && tkey && tlag && tm && twa && twc && lam && musum &&
                   munext && ek && muout && ga,
               "spr_exchange_adjoint_ds: null pointer");
   SPR_REQUIRE(muout != munext && muout != lam && muout != ek,
@@ -358,10 +357,12 @@ extern "C" int spr_exchange_adjoint_ds(
   const int64_t nb = P * B * nt;
   SPR_REQUIRE(nb < (int64_t)0x7fffffff, "spr_exchange_adjoint_ds: grid too "
                                         "large");
+  hipStream_t s = as_stream(stream);
+  SPR_HIP(hipMemcpyAsync(muout, lam, sizeof(float) * P * D * B * T,
+                         hipMemcpyDeviceToDevice, s));
   hipLaunchKernelGGL((adjoint_ds_kernel<R, C>), dim3((unsigned)nb), dim3(256),
-                     0, as_stream(stream), (int)D, (int)B, (int)T, (int)nt,
-                     nb, Vd, tgptr, tj, tlag, tm, twa, twc, lam, musum, munext,
-                     ek, muout, ga);
+                     0, s, (int)D, (int)B, (int)T, (int)nt, nb, Vd, tgptr,
+                     tkey, tlag, tm, twa, twc, musum, munext, ek, muout, ga);
   SPR_LAUNCH_CHECK();
   return SPR_OK;
 }
