@@ -307,6 +307,9 @@ def main(argv=None):
                     choices=["auto", "ds", "grouped", "generic"],
                     help="exchange plan (auto: grouped when sources share "
                          "BRDF rows)")
+    ap.add_argument("--tile-rows", type=int, default=None, choices=[0, 4, 8],
+                    help="destination rows per tile of the grouped gather "
+                         "(default: the class default; 0: row by row)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a device copy of known size (to scale "
                          "the FETCH_SIZE/WRITE_SIZE counters)")
@@ -345,6 +348,8 @@ def main(argv=None):
     for idx, br in sc.brdf_groups:
         r.set_wall_brdf(idx, br, sc.directions, sc.directions)
     r.set_air_attenuation(FrequencyData(sc.air_attenuation, sc.frequencies))
+    if args.tile_rows is not None:
+        r.TILE_ROWS = args.tile_rows
     t["host_setup_s"] = tic() - t0
     t0 = tic()
     r.bake_geometry()
@@ -360,7 +365,10 @@ def main(argv=None):
     kind = ("generic" if not isinstance(plan, ops.GroupedPlan) else
             "ds" if plan.diffuse_specular else "grouped")
     t["plan_s"] = tic() - t0
-    log(f"plan {t['plan_s']:.2f} s ({kind}), directed pairs={plan.Vd}")
+    tiles = getattr(plan, "tiles", None)
+    log(f"plan {t['plan_s']:.2f} s ({kind}), directed pairs={plan.Vd}"
+        + (f", tiles of {tiles.JT} rows: {tiles.M} merged entries"
+           if tiles is not None else ""))
 
     e0d = r._dev["energy_init_source"]
     d0 = r._dev["distance_patches_to_source"]
@@ -427,15 +435,17 @@ def main(argv=None):
         gather_bytes = 4 * U + Vd * (8 + 8 * B * n_w) + 4 * G * B * T
         # spread: read the group sums, write cur, read+write hi and lo
         spread_bytes = 4 * G * B * T + 20 * n_state
+        gather_name = ("tile_gather_kernel" if tiles is not None else
+                       "group_gather_kernel")
         kernels = {
-            "group_gather_kernel": {
+            gather_name: {
                 "ms": gather_s * 1e3, "algorithmic_bytes": gather_bytes,
                 "GB/s": gather_bytes / gather_s / 1e9},
             "group_spread_kernel": {
                 "ms": spread_s * 1e3, "algorithmic_bytes": spread_bytes,
                 "GB/s": spread_bytes / spread_s / 1e9},
         }
-        dom, dom_s, dom_bytes = "group_gather_kernel", gather_s, gather_bytes
+        dom, dom_s, dom_bytes = gather_name, gather_s, gather_bytes
     else:
         dom, dom_s, dom_bytes = "exchange_step_kernel", step_s, order_bytes
     achieved_gbs = dom_bytes / dom_s / 1e9
@@ -460,6 +470,8 @@ def main(argv=None):
             "directed_pairs": Vd, "updates_per_order": U,
             "plan": kind,
             "groups": plan.n_groups if two_pass else None,
+            "tile_rows": tiles.JT if tiles is not None else 0,
+            "merged_entries": tiles.M if tiles is not None else None,
         },
         "roofline": {
             "bound": "hbm", "achieved": achieved_gbs, "peak": HBM_PEAK_GBS,

@@ -327,6 +327,43 @@ int spr_exchange_gather_ds(int64_t P, int64_t B, int64_t T, int64_t Vd,
                            const double *scal, const double *dscal,
                            const float *prev, float *gsum, void *stream);
 
+/* Destination tiles of the grouped plan (the _energy_exchange loop,
+ * RadiosityFast.py:1073-1145, as grouped above).  A tile is JT (4 or 8)
+ * consecutive destination rows; its entries are merged into distinct
+ * (group key, source row) entries, in key then source-row order, each with
+ * one delay word and one weight per slot (row of the tile), so the JT
+ * gathers of one source row are issued together and share cache lines.
+ *
+ * Pass 1: skey[Vd], sorted[Vd], payload[Vd] (int64 workspace), first[Vd+1]
+ * (int64: exclusive count of distinct entries before each merged position;
+ * first[Vd] = M, the number of merged entries) and tptr[ceil(P/JT)+1]
+ * (merged CSR of the tiles). */
+int spr_plan_tile_order(int64_t P, int64_t JT, int64_t Vd, int64_t G,
+                        const int64_t *row_ptr, const int64_t *grp_start,
+                        const int32_t *grp_key, const int32_t *rowid,
+                        int64_t *skey, int64_t *sorted, int64_t *payload,
+                        int64_t *first, int64_t *tptr, void *stream);
+
+/* Pass 2: mrow[M] (int32 source row), mlag[M,JT] (int32 delay per slot,
+ * 0x3fffffff for a slot without that entry, bit 30 set where the slot's
+ * group ends: on the last merged entry of the key run) and mw[B,M,JT]
+ * (float64 weight per band and slot, 0 for a slot without the entry). */
+int spr_plan_tile_fill(int64_t P, int64_t JT, int64_t B, int64_t Vd,
+                       int64_t M, const int64_t *row_ptr,
+                       const int32_t *rowid, const int32_t *lag,
+                       const double *scal, const int64_t *sorted,
+                       const int64_t *payload, const int64_t *first,
+                       int32_t *mrow, int32_t *mlag, double *mw, void *stream);
+
+/* Gather pass on the tiles: gsum bit-identical to
+ * spr_exchange_gather_grouped (each slot sums its groups in the same
+ * order), for use with spr_exchange_spread_grouped. */
+int spr_exchange_gather_tiled(int64_t P, int64_t JT, int64_t B, int64_t T,
+                              int64_t M, const int64_t *tptr,
+                              const int64_t *grp_ptr, const int32_t *mrow,
+                              const int32_t *mlag, const double *mw,
+                              const float *prev, float *gsum, void *stream);
+
 /* etc[P,D,B,T] (float32) = tot_hi + tot_lo (may alias tot_hi). */
 int spr_exchange_finalize(int64_t n, const float *tot_hi,
                           const float *tot_lo, float *etc, void *stream);

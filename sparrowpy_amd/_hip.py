@@ -96,6 +96,13 @@ SYMBOLS = {
                                 _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i64,
                                 _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
                                 _vp, _vp, _vp, _vp, _vp]),
+    "spr_plan_tile_order": (_int, [_i64, _i64, _i64, _i64, _vp, _vp, _vp,
+                                   _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "spr_plan_tile_fill": (_int, [_i64, _i64, _i64, _i64, _i64, _vp, _vp,
+                                  _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
+                                  _vp]),
+    "spr_exchange_gather_tiled": (_int, [_i64, _i64, _i64, _i64, _i64, _vp,
+                                         _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "spr_exchange_gather_ds": (_int, [_i64, _i64, _i64, _i64, _vp, _vp, _vp,
                                       _vp, _vp, _vp, _vp, _vp, _vp]),
     "spr_ds_weights": (_int, [_i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),

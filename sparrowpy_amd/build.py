@@ -17,8 +17,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIBDIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIBDIR, "libsparrow_hip.so")
-SOURCES = ["util.hip", "bake.hip", "exchange.hip", "collect.hip",
-           "adjoint.hip"]
+SOURCES = ["util.hip", "bake.hip", "exchange.hip", "tiles.hip",
+           "collect.hip", "adjoint.hip"]
 ARCH = os.environ.get("SPARROW_HIP_ARCH", "gfx950")
 
 HIPFLAGS = [

@@ -1,0 +1,422 @@
+// Destination tiles of the grouped plan.
+//
+// Reference: the per-order loop of _energy_exchange
+// (sparrowpy/classes/RadiosityFast.py:1073-1145), as grouped by
+// spr_plan_grouped_sort / spr_plan_grouped_fill.
+//
+// Neighbouring destination patches (consecutive patch ids on one wall) see
+// nearly the same sources through nearly the same outgoing directions, so
+// their plan rows read nearly the same source rows prev[i, d, b, :], only
+// shifted by slightly different delays.  A tile of JT consecutive rows is
+// merged into one list of distinct (group key, source row) entries, each
+// carrying one (delay, weight) per slot (row of the tile): a workgroup then
+// issues the JT slots' gathers of one source row back to back, so only the
+// first misses in the caches and the rest hit the lines it brought in.  On
+// the street-canyon plan JT = 4 merges 16.4 M entries into 5.2 M.
+//
+// Within a key run the merged list is in ascending source row order, the
+// order of the entries of each group in the grouped plan, so every slot sums
+// its group in exactly the grouped plan's order: the per-group sums are
+// bit-identical to spr_exchange_gather_grouped's.
+#include "spr_common.hpp"
+#include "spr_exchange.hpp"
+#include "spr_util.hpp"
+
+namespace spr {
+namespace {
+
+// Largest k in [lo, hi) with a[k] <= x (a ascending, a[lo] <= x).
+__device__ __forceinline__ int64_t last_le(const int64_t *__restrict__ a,
+                                           int64_t lo, int64_t hi,
+                                           int64_t x) {
+  while (hi - lo > 1) {
+    const int64_t mid = lo + (hi - lo) / 2;
+    if (a[mid] <= x)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+// Number of k in [lo, hi) with a[k] < x (strict) or a[k] <= x (!strict).
+__device__ __forceinline__ int64_t count_below(const int64_t *__restrict__ a,
+                                               int64_t lo, int64_t hi,
+                                               int64_t x, bool strict) {
+  const int64_t base = lo;
+  while (lo < hi) {
+    const int64_t mid = lo + (hi - lo) / 2;
+    if (strict ? a[mid] < x : a[mid] <= x)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  return lo - base;
+}
+
+// skey[e] = (group key << 32) | source row: ascending within every plan row
+// (groups in key order, entries of a group in source order).
+__global__ __launch_bounds__(256) void tile_skey_kernel(
+    int64_t Vd, int64_t G, const int64_t *__restrict__ grp_start,
+    const int32_t *__restrict__ grp_key, const int32_t *__restrict__ rowid,
+    int64_t *__restrict__ skey) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= Vd) return;
+  const int64_t g = last_le(grp_start, 0, G, e);
+  skey[e] = ((int64_t)grp_key[g] << 32) | (int64_t)(uint32_t)rowid[e];
+}
+
+// Merge of the JT sorted rows of each tile: entry e of row j (slot
+// k = j mod JT) goes to position pos = row_ptr[tile start] + (its rank in
+// its row) + (entries of the other rows of the tile that sort before it,
+// ties ordered by slot).  A tile's positions are the span of its rows.
+__global__ __launch_bounds__(256) void tile_merge_kernel(
+    int64_t P, int JT, int64_t Vd, const int64_t *__restrict__ row_ptr,
+    const int64_t *__restrict__ skey, int64_t *__restrict__ sorted,
+    int64_t *__restrict__ payload) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= Vd) return;
+  const int64_t j = last_le(row_ptr, 0, P, e);
+  const int k = (int)(j % JT);
+  const int64_t j0 = j - k;
+  const int64_t s = skey[e];
+  int64_t pos = row_ptr[j0] + (e - row_ptr[j]);
+  for (int q = 0; q < JT; ++q) {
+    const int64_t jq = j0 + q;
+    if (q == k || jq >= P) continue;
+    pos += count_below(skey, row_ptr[jq], row_ptr[jq + 1], s, q > k);
+  }
+  sorted[pos] = s;
+  payload[pos] = (e << 4) | k;
+}
+
+__device__ __forceinline__ bool run_start(const int64_t *__restrict__ sorted,
+                                          int64_t p, int64_t tile_begin) {
+  return p == tile_begin || sorted[p] != sorted[p - 1];
+}
+
+__device__ __forceinline__ int64_t tile_begin_of(
+    int64_t P, int JT, const int64_t *__restrict__ row_ptr, int64_t p) {
+  const int64_t j = last_le(row_ptr, 0, P, p);
+  return row_ptr[j - j % JT];
+}
+
+// first[p] = 1 where a distinct (key, source row) starts in its tile.
+__global__ __launch_bounds__(256) void tile_first_kernel(
+    int64_t P, int JT, int64_t Vd, const int64_t *__restrict__ row_ptr,
+    const int64_t *__restrict__ sorted, int64_t *__restrict__ first) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= Vd) return;
+  first[p] = run_start(sorted, p, tile_begin_of(P, JT, row_ptr, p)) ? 1 : 0;
+}
+
+// tptr[t] = merged index of tile t's first entry (first = exclusive scan).
+__global__ void tile_ptr_kernel(int64_t P, int JT, int64_t n_tiles,
+                                const int64_t *__restrict__ row_ptr,
+                                const int64_t *__restrict__ first,
+                                int64_t *__restrict__ tptr) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t > n_tiles) return;
+  const int64_t j = t * JT < P ? t * JT : P;
+  tptr[t] = first[row_ptr[j]];
+}
+
+__device__ __forceinline__ int64_t merged_index(
+    const int64_t *__restrict__ sorted, const int64_t *__restrict__ first,
+    int64_t p, int64_t tile_begin) {
+  return run_start(sorted, p, tile_begin) ? first[p] : first[p] - 1;
+}
+
+// Merged entry of position p: its source row, and slot k's delay (without
+// flag) and per-band weights.  mlag starts at kLagMask, mw at 0 (slots a
+// merged entry does not have).
+__global__ __launch_bounds__(256) void tile_fill_kernel(
+    int64_t P, int JT, int64_t B, int64_t Vd, int64_t M,
+    const int64_t *__restrict__ row_ptr, const int32_t *__restrict__ rowid,
+    const int32_t *__restrict__ lag, const double *__restrict__ scal,
+    const int64_t *__restrict__ sorted, const int64_t *__restrict__ payload,
+    const int64_t *__restrict__ first, int32_t *__restrict__ mrow,
+    int32_t *__restrict__ mlag, double *__restrict__ mw) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= Vd) return;
+  const int64_t m =
+      merged_index(sorted, first, p, tile_begin_of(P, JT, row_ptr, p));
+  const int64_t e = payload[p] >> 4;
+  const int k = (int)(payload[p] & 15);
+  mrow[m] = rowid[e];
+  mlag[m * JT + k] = lag[e] & kLagMask;
+  for (int64_t b = 0; b < B; ++b) mw[(b * M + m) * JT + k] = scal[b * Vd + e];
+}
+
+// Group ends: the entry flagged last of its group in slot k's row sets the
+// flag on slot k of the last merged entry of its key run in the tile, where
+// the tile kernel stores slot k's sum and moves to its next group.
+__global__ __launch_bounds__(256) void tile_flag_kernel(
+    int64_t P, int JT, int64_t Vd, const int64_t *__restrict__ row_ptr,
+    const int32_t *__restrict__ lag, const int64_t *__restrict__ sorted,
+    const int64_t *__restrict__ payload, const int64_t *__restrict__ first,
+    int32_t *__restrict__ mlag) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= Vd) return;
+  const int64_t e = payload[p] >> 4;
+  if (!(lag[e] & kLastInGroup)) return;
+  const int k = (int)(payload[p] & 15);
+  const int64_t j = last_le(row_ptr, 0, P, p);
+  const int64_t tb = row_ptr[j - j % JT];
+  const int64_t jt_end = j - j % JT + JT < P ? j - j % JT + JT : P;
+  const int64_t te = row_ptr[jt_end];
+  // last position of the tile whose key equals this entry's key
+  const int64_t key_end = ((sorted[p] >> 32) + 1) << 32;
+  const int64_t q = p + count_below(sorted, p, te, key_end, true) - 1;
+  const int64_t m = merged_index(sorted, first, q, tb);
+  mlag[m * JT + k] |= kLastInGroup;
+}
+
+// ---------------------------------------------------------------------
+// the tiled gather
+// ---------------------------------------------------------------------
+// Merged entries are walked in blocks of MB: one vector load per field
+// brings a block's metadata (source row, JT delay words, JT weights), one
+// entry per lane, broadcast with v_readlane.  A block is fully unrolled
+// into MB/C chunks of C merged entries; chunk q+1's C*JT*R gathers are in
+// flight while chunk q is summed.  A slot without an entry has delay
+// kLagMask (a descriptor of size 0: loads give 0) and weight 0.
+// Requires T < 2^29.
+template <int JT>
+struct TileMeta {
+  int32_t row;
+  int32_t lag[JT];
+  double w[JT];
+};
+
+template <int JT>
+__device__ __forceinline__ TileMeta<JT> load_tile_meta(
+    int64_t m0, int64_t m_end, const int32_t *__restrict__ mrow,
+    const int32_t *__restrict__ mlag, const double *__restrict__ w) {
+  const int64_t m = m0 + (int64_t)(threadIdx.x & 63);
+  const bool in = m < m_end;
+  const int64_t mc = in ? m : m_end - 1;
+  TileMeta<JT> t;
+  t.row = mrow[mc];
+#pragma unroll
+  for (int k = 0; k < JT; ++k) {
+    t.lag[k] = in ? mlag[mc * JT + k] : kLagMask;
+    t.w[k] = in ? w[mc * JT + k] : 0.0;
+  }
+  return t;
+}
+
+template <int R, int BS, int C, int JT, int MB>
+__global__ __launch_bounds__(BS) void tile_gather_kernel(
+    int64_t P, int B, int T, int n_tblk, int64_t nblocks, int64_t M,
+    const int64_t *__restrict__ tptr, const int64_t *__restrict__ grp_ptr,
+    const int32_t *__restrict__ mrow, const int32_t *__restrict__ mlag,
+    const double *__restrict__ mw, const float *__restrict__ prev,
+    float *__restrict__ gsum) {
+  static_assert(MB <= 64 && MB % (2 * C) == 0, "chunks pair up in a block");
+  constexpr int NQ = MB / C;
+  int64_t tile;
+  int b, tb;
+  dest_major_tile(xcd_tile(blockIdx.x, nblocks), nblocks / (B * n_tblk),
+                  n_tblk, tile, b, tb);
+  const int t0 = tb * (BS * R);
+  const int t_end = min(T, t0 + BS * R);
+  const int64_t m_begin = tptr[tile], m_end = tptr[tile + 1];
+  if (m_end <= m_begin) return;
+  const double *w = mw + (int64_t)b * M * JT;
+  const uint32_t row_bytes = (uint32_t)T * 4u;
+  const int tl = t0 + (int)threadIdx.x;
+  int64_t g[JT];
+#pragma unroll
+  for (int k = 0; k < JT; ++k) {
+    const int64_t j = tile * JT + k;
+    g[k] = j < P ? grp_ptr[j] : 0;
+  }
+
+  auto fetch = [&](const TileMeta<JT> &m, int q, float (&v)[C][JT][R]) {
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+      const int idx = q * C + c;
+      const int32_t rk = __builtin_amdgcn_readlane(m.row, idx);
+      const float *src = prev + ((int64_t)rk * B + b) * T;
+#pragma unroll
+      for (int k = 0; k < JT; ++k) {
+        const int n = min(__builtin_amdgcn_readlane(m.lag[k], idx) & kLagMask,
+                          T);
+        const __amdgpu_buffer_rsrc_t rs =
+            make_rsrc(src, n < t_end ? row_bytes : 0u);
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+          v[c][k][r] = buffer_load(rs, (uint32_t)(tl + BS * r - n) * 4u);
+      }
+    }
+  };
+
+  double S[JT][R];
+#pragma unroll
+  for (int k = 0; k < JT; ++k)
+#pragma unroll
+    for (int r = 0; r < R; ++r) S[k][r] = 0.0;
+  auto consume = [&](const TileMeta<JT> &m, int q,
+                     const float (&v)[C][JT][R]) {
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+      const int idx = q * C + c;
+#pragma unroll
+      for (int k = 0; k < JT; ++k) {
+        const double wk = readlane_f64(m.w[k], idx);
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+          S[k][r] = fma(wk, (double)v[c][k][r], S[k][r]);
+      }
+#pragma unroll
+      for (int k = 0; k < JT; ++k) {
+        if (__builtin_amdgcn_readlane(m.lag[k], idx) & kLastInGroup) {
+          const __amdgpu_buffer_rsrc_t os =
+              make_rsrc(gsum + (g[k] * B + b) * (int64_t)T, row_bytes);
+#pragma unroll
+          for (int r = 0; r < R; ++r) {
+            buffer_store((float)S[k][r], os, (uint32_t)(tl + BS * r) * 4u);
+            S[k][r] = 0.0;
+          }
+          g[k] += 1;
+        }
+      }
+    }
+  };
+
+  TileMeta<JT> cur = load_tile_meta<JT>(m_begin, m_end, mrow, mlag, w);
+  TileMeta<JT> next =
+      load_tile_meta<JT>(m_begin + MB, m_end, mrow, mlag, w);
+  float va[C][JT][R], vb[C][JT][R];
+  fetch(cur, 0, va);
+  for (int64_t blk = m_begin; blk < m_end; blk += MB) {
+#pragma unroll
+    for (int q = 0; q < NQ; q += 2) {
+      fetch(cur, q + 1, vb);
+      consume(cur, q, va);
+      if (q + 2 < NQ)
+        fetch(cur, q + 2, va);
+      else
+        fetch(next, 0, va);  // first chunk of the next block
+      consume(cur, q + 1, vb);
+    }
+    cur = next;
+    next = load_tile_meta<JT>(blk + 2 * MB, m_end, mrow, mlag, w);
+  }
+}
+
+int launch_tile_gather(int64_t P, int JT, int64_t B, int64_t T, int64_t M,
+                       const int64_t *tptr, const int64_t *grp_ptr,
+                       const int32_t *mrow, const int32_t *mlag,
+                       const double *mw, const float *prev, float *gsum,
+                       hipStream_t s) {
+  constexpr int BS = 256, R = 2;
+  const int64_t n_tiles = ceil_div(P, JT);
+  const int64_t nt = ceil_div(T, (int64_t)BS * R);
+  const int64_t nb = n_tiles * B * nt;
+  SPR_REQUIRE(nb < (int64_t)0x7fffffff, "exchange grid too large");
+  SPR_REQUIRE(T < ((int64_t)1 << 29), "histogram too long (T >= 2^29)");
+  if (JT == 4)
+    hipLaunchKernelGGL((tile_gather_kernel<R, BS, 2, 4, 64>),
+                       dim3((unsigned)nb), dim3(BS), 0, s, P, (int)B, (int)T,
+                       (int)nt, nb, M, tptr, grp_ptr, mrow, mlag, mw, prev,
+                       gsum);
+  else
+    hipLaunchKernelGGL((tile_gather_kernel<R, BS, 1, 8, 32>),
+                       dim3((unsigned)nb), dim3(BS), 0, s, P, (int)B, (int)T,
+                       (int)nt, nb, M, tptr, grp_ptr, mrow, mlag, mw, prev,
+                       gsum);
+  SPR_LAUNCH_CHECK();
+  return SPR_OK;
+}
+
+unsigned grid_of(int64_t n) { return (unsigned)ceil_div(n > 0 ? n : 1, 256); }
+
+}  // namespace
+}  // namespace spr
+
+using namespace spr;
+
+extern "C" int spr_plan_tile_order(int64_t P, int64_t JT, int64_t Vd,
+                                   int64_t G, const int64_t *row_ptr,
+                                   const int64_t *grp_start,
+                                   const int32_t *grp_key,
+                                   const int32_t *rowid, int64_t *skey,
+                                   int64_t *sorted, int64_t *payload,
+                                   int64_t *first, int64_t *tptr,
+                                   void *stream) {
+  clear_error();
+  SPR_REQUIRE(P > 0 && Vd >= 0 && G >= 0 && (JT == 4 || JT == 8),
+              "spr_plan_tile_order: bad sizes (JT must be 4 or 8)");
+  SPR_REQUIRE(Vd < ((int64_t)1 << 40), "spr_plan_tile_order: plan too large");
+  SPR_REQUIRE(row_ptr && tptr && first, "spr_plan_tile_order: null pointer");
+  SPR_REQUIRE(Vd == 0 || (grp_start && grp_key && rowid && skey && sorted &&
+                          payload && G > 0),
+              "spr_plan_tile_order: null pointer");
+  hipStream_t s = as_stream(stream);
+  const int64_t n_tiles = ceil_div(P, JT);
+  if (Vd > 0) {
+    hipLaunchKernelGGL(tile_skey_kernel, dim3(grid_of(Vd)), dim3(256), 0, s,
+                       Vd, G, grp_start, grp_key, rowid, skey);
+    SPR_LAUNCH_CHECK();
+    hipLaunchKernelGGL(tile_merge_kernel, dim3(grid_of(Vd)), dim3(256), 0, s,
+                       P, (int)JT, Vd, row_ptr, skey, sorted, payload);
+    SPR_LAUNCH_CHECK();
+    hipLaunchKernelGGL(tile_first_kernel, dim3(grid_of(Vd)), dim3(256), 0, s,
+                       P, (int)JT, Vd, row_ptr, sorted, first);
+    SPR_LAUNCH_CHECK();
+  }
+  int rc = scan_exclusive(first, Vd, s);
+  if (rc != SPR_OK) return rc;
+  hipLaunchKernelGGL(tile_ptr_kernel, dim3(grid_of(n_tiles + 1)), dim3(256),
+                     0, s, P, (int)JT, n_tiles, row_ptr, first, tptr);
+  SPR_LAUNCH_CHECK();
+  return SPR_OK;
+}
+
+extern "C" int spr_plan_tile_fill(int64_t P, int64_t JT, int64_t B,
+                                  int64_t Vd, int64_t M,
+                                  const int64_t *row_ptr,
+                                  const int32_t *rowid, const int32_t *lag,
+                                  const double *scal, const int64_t *sorted,
+                                  const int64_t *payload,
+                                  const int64_t *first, int32_t *mrow,
+                                  int32_t *mlag, double *mw, void *stream) {
+  clear_error();
+  SPR_REQUIRE(P > 0 && B > 0 && Vd >= 0 && M >= 0 && M <= Vd &&
+                  (JT == 4 || JT == 8),
+              "spr_plan_tile_fill: bad sizes (JT must be 4 or 8)");
+  if (M == 0) return SPR_OK;
+  SPR_REQUIRE(row_ptr && rowid && lag && scal && sorted && payload && first &&
+                  mrow && mlag && mw,
+              "spr_plan_tile_fill: null pointer");
+  hipStream_t s = as_stream(stream);
+  int rc = fill<int32_t>(mlag, M * JT, kLagMask, s);
+  if (rc == SPR_OK) rc = fill<double>(mw, B * M * JT, 0.0, s);
+  if (rc != SPR_OK) return rc;
+  hipLaunchKernelGGL(tile_fill_kernel, dim3(grid_of(Vd)), dim3(256), 0, s, P,
+                     (int)JT, B, Vd, M, row_ptr, rowid, lag, scal, sorted,
+                     payload, first, mrow, mlag, mw);
+  SPR_LAUNCH_CHECK();
+  hipLaunchKernelGGL(tile_flag_kernel, dim3(grid_of(Vd)), dim3(256), 0, s, P,
+                     (int)JT, Vd, row_ptr, lag, sorted, payload, first, mlag);
+  SPR_LAUNCH_CHECK();
+  return SPR_OK;
+}
+
+extern "C" int spr_exchange_gather_tiled(int64_t P, int64_t JT, int64_t B,
+                                         int64_t T, int64_t M,
+                                         const int64_t *tptr,
+                                         const int64_t *grp_ptr,
+                                         const int32_t *mrow,
+                                         const int32_t *mlag,
+                                         const double *mw, const float *prev,
+                                         float *gsum, void *stream) {
+  SPR_REQUIRE(P > 0 && B > 0 && T > 0 && M >= 0 && (JT == 4 || JT == 8),
+              "spr_exchange_gather_tiled: bad sizes (JT must be 4 or 8)");
+  SPR_REQUIRE(tptr && grp_ptr && mrow && mlag && mw && prev && gsum,
+              "spr_exchange_gather_tiled: null pointer");
+  return launch_tile_gather(P, (int)JT, B, T, M, tptr, grp_ptr, mrow, mlag,
+                            mw, prev, gsum, as_stream(stream));
+}

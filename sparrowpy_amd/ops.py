@@ -376,6 +376,7 @@ class GroupedPlan:
     scal: torch.Tensor
     rtab: torch.Tensor
     dscal: torch.Tensor = None  # diffuse weights (diffuse + specular plan)
+    tiles: "TilePlan" = None    # merged destination tiles (tile_plan)
 
     @property
     def Vd(self):
@@ -440,6 +441,52 @@ class GroupedPlan:
              stream_ptr())
         return cls(P, D, B, row_ptr, grp_ptr, grp_start, grp_key,
                    rowid[:Vd], lag[:Vd], scal[:, :Vd].contiguous(), rtab)
+
+
+@dataclasses.dataclass
+class TilePlan:
+    """Destination tiles of a GroupedPlan: JT consecutive rows merged into
+    M distinct (group key, source row) entries with one delay word and one
+    weight per slot; ``tptr`` is the tiles' CSR over the merged entries,
+    ``mlag[M, JT]`` and ``mw[B, M, JT]`` (see sparrow_hip.h)."""
+
+    JT: int
+    M: int
+    tptr: torch.Tensor
+    mrow: torch.Tensor
+    mlag: torch.Tensor
+    mw: torch.Tensor
+
+
+def tile_plan(plan, jt=4, min_merge=1.5):
+    """TilePlan of a (non diffuse + specular) GroupedPlan, or None when
+    tiles of ``jt`` rows merge fewer than ``min_merge`` entries into one."""
+    if plan.diffuse_specular:
+        raise ValueError("tiles are built for the grouped plan only")
+    jt = int(jt)
+    P, B, Vd, G = plan.P, plan.B, plan.Vd, plan.n_groups
+    dev = plan.row_ptr.device
+    n_tiles = -(-P // jt)
+    tptr = torch.empty(n_tiles + 1, dtype=torch.int64, device=dev)
+    first = torch.empty(Vd + 1, dtype=torch.int64, device=dev)
+    work = [torch.empty(max(Vd, 1), dtype=torch.int64, device=dev)
+            for _ in range(3)]
+    skey, srt, payload = work
+    call("spr_plan_tile_order", P, jt, Vd, G, ptr(plan.row_ptr),
+         ptr(plan.grp_start), ptr(plan.grp_key), ptr(plan.rowid), ptr(skey),
+         ptr(srt), ptr(payload), ptr(first), ptr(tptr), stream_ptr())
+    M = int(first[Vd].item())
+    if M == 0 or Vd < min_merge * M:
+        return None
+    del skey
+    mrow = torch.empty(M, dtype=torch.int32, device=dev)
+    mlag = torch.empty((M, jt), dtype=torch.int32, device=dev)
+    mw = torch.empty((B, M, jt), dtype=torch.float64, device=dev)
+    call("spr_plan_tile_fill", P, jt, B, Vd, M, ptr(plan.row_ptr),
+         ptr(plan.rowid), ptr(plan.lag), ptr(plan.scal), ptr(srt),
+         ptr(payload), ptr(first), ptr(mrow), ptr(mlag), ptr(mw),
+         stream_ptr())
+    return TilePlan(jt, M, tptr, mrow, mlag, mw)
 
 
 def diffuse_specular_tables(scattering, n_directions, n_bins, n_rows):
@@ -586,6 +633,13 @@ class ExchangeState:
                  ptr(plan.row_ptr), ptr(plan.grp_ptr), ptr(plan.rowid),
                  ptr(plan.lag), ptr(plan.scal), ptr(plan.dscal),
                  ptr(self.prev), ptr(self._gsum(plan)), stream_ptr())
+            return
+        if plan.tiles is not None:
+            tp = plan.tiles
+            call("spr_exchange_gather_tiled", P, tp.JT, B, T, tp.M,
+                 ptr(tp.tptr), ptr(plan.grp_ptr), ptr(tp.mrow), ptr(tp.mlag),
+                 ptr(tp.mw), ptr(self.prev), ptr(self._gsum(plan)),
+                 stream_ptr())
             return
         call("spr_exchange_gather_grouped", P, B, T, plan.Vd,
              ptr(plan.row_ptr), ptr(plan.grp_ptr), ptr(plan.rowid),

@@ -471,6 +471,11 @@ class DirectionalRadiosityFast:
     # Use the grouped plan when groups are at most this fraction of pairs
     # (each group costs D FMAs per bin, each pair one FMA).
     GROUPED_MAX_GROUP_FRACTION = 0.25
+    # Rows per destination tile of the grouped plan's gather (4 or 8; 0
+    # gathers row by row); tiles are used when they merge at least
+    # TILE_MIN_MERGE entries into one.
+    TILE_ROWS = 4
+    TILE_MIN_MERGE = 1.5
 
     def _exchange_plan(self, speed_of_sound, time_resolution, kind=None):
         """Plan of the exchange (cached per speed of sound and bin width).
@@ -480,7 +485,8 @@ class DirectionalRadiosityFast:
         share few BRDF rows, else "ds" when every BRDF row is a constant
         plus one specular entry, else "generic").
         """
-        key = (float(speed_of_sound), float(time_resolution), kind)
+        key = (float(speed_of_sound), float(time_resolution), kind,
+               self.TILE_ROWS, self.TILE_MIN_MERGE)
         if getattr(self, "_plan", None) is not None and self._plan[0] == key:
             return self._plan[1]
         b = self._bake
@@ -511,6 +517,9 @@ class DirectionalRadiosityFast:
             if (plan is not None and kind is None and plan.Vd and
                     plan.n_groups > self.GROUPED_MAX_GROUP_FRACTION * plan.Vd):
                 plan = None
+            if plan is not None and self.TILE_ROWS:
+                plan.tiles = ops.tile_plan(plan, self.TILE_ROWS,
+                                           self.TILE_MIN_MERGE)
         if plan is None and kind in (None, "ds"):
             try:
                 plan = ops.diffuse_specular_plan(*args)

@@ -1,0 +1,179 @@
+"""Destination tiles of the grouped plan (spr_plan_tile_*,
+spr_exchange_gather_tiled).
+
+Bar: the merged plan holds exactly the grouped plan's entries, slot by
+slot and in the same order; the tiled gather's group sums are bit-identical
+to the row-by-row gather's (each slot sums its groups in the same order);
+the exchange on tiles meets the oracle at the exchange tolerance.
+"""
+import numpy as np
+import pytest
+
+import oracle as orc
+from conftest import make_oracle, rel_err_within_db
+from sparrowpy_amd import testing
+
+from test_gpu_parity import _directional, _grouped_and_generic
+
+pytestmark = pytest.mark.gpu
+
+EXCHANGE_RTOL = 1e-5
+LAG_MASK = 0x3fffffff
+
+
+@pytest.fixture(scope="module")
+def ops():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from sparrowpy_amd import ops as _ops
+    return _ops
+
+
+def _np(t):
+    return t.detach().cpu().numpy()
+
+
+# 148 patches (a partial tile of 8) and 62 (partial tiles of 4 and 8)
+ROOMS = {"3x2.5x2": (3, 2.5, 2), "2.5x1.5x1": (2.5, 1.5, 1)}
+
+
+def _room(name, per_wall=True, seed=3):
+    r = make_oracle(testing.shoebox_room_stub(*ROOMS[name]), .5)
+    _directional(r, per_wall=per_wall, seed=seed)
+    r.bake_geometry()
+    return r
+
+
+@pytest.mark.parametrize("jt", [4, 8])
+@pytest.mark.parametrize("name", list(ROOMS))
+def test_tile_plan_contents(ops, name, jt):
+    r = _room(name)
+    gp, _ = _grouped_and_generic(ops, r, 343., 1 / 2000)
+    tp = ops.tile_plan(gp, jt, min_merge=0)
+    P, B = gp.P, gp.B
+    flag = ops.GroupedPlan.LAST_IN_GROUP
+    rp, gs, gptr = _np(gp.row_ptr), _np(gp.grp_start), _np(gp.grp_ptr)
+    rowid, lag, scal = _np(gp.rowid), _np(gp.lag), _np(gp.scal)
+    tptr, mrow = _np(tp.tptr), _np(tp.mrow)
+    mlag, mw = _np(tp.mlag), _np(tp.mw)
+    n_tiles = -(-P // jt)
+    assert tptr.shape == (n_tiles + 1,) and tptr[0] == 0
+    assert tptr[-1] == tp.M and np.all(np.diff(tptr) >= 0)
+    # M = number of distinct (tile, group key, source row)
+    g_of_e = np.searchsorted(gs, np.arange(gp.Vd), side="right") - 1
+    j_of_e = np.searchsorted(rp, np.arange(gp.Vd), side="right") - 1
+    key = _np(gp.grp_key)[g_of_e].astype(np.int64)
+    distinct = np.unique(((j_of_e // jt) << 44) | (key << 22) |
+                         rowid.astype(np.int64))
+    assert distinct.size == tp.M
+    assert tp.M < gp.Vd
+    for t in range(n_tiles):
+        ms = slice(tptr[t], tptr[t + 1])
+        for k in range(jt):
+            j = t * jt + k
+            words = mlag[ms, k]
+            present = (words & LAG_MASK) != LAG_MASK
+            if j >= P:
+                assert not present.any() and not (words & flag).any()
+                assert not mw[:, ms, k].any()
+                continue
+            es = slice(rp[j], rp[j + 1])
+            # the slot's entries, in merged order, are the row's, in order
+            np.testing.assert_array_equal(mrow[ms][present], rowid[es])
+            np.testing.assert_array_equal(words[present] & ~flag,
+                                          lag[es] & LAG_MASK)
+            np.testing.assert_array_equal(mw[:, ms, k][:, present],
+                                          scal[:, es])
+            assert not mw[:, ms, k][:, ~present].any()
+            # a group ends where the slot has seen exactly its entries
+            seen = np.cumsum(present)[(words & flag) != 0]
+            ends = gs[gptr[j] + 1:gptr[j + 1] + 1] - rp[j]
+            np.testing.assert_array_equal(seen, ends)
+
+
+def _random_state(ops, P, D, B, T, seed):
+    import torch
+    g = torch.Generator(device="cuda").manual_seed(seed)
+    st = ops.ExchangeState(P, D, B, T)
+    st.prev.copy_(torch.rand(st.prev.shape, generator=g, device="cuda"))
+    return st
+
+
+@pytest.mark.parametrize("jt", [4, 8])
+@pytest.mark.parametrize("per_wall", [True, False])
+@pytest.mark.parametrize("name", list(ROOMS))
+def test_tiled_gather_bitexact(ops, name, per_wall, jt):
+    """Group sums on tiles equal the row-by-row gather's bit for bit, for
+    histograms spanning several time blocks (T = 1,300 > 512)."""
+    import torch
+    r = _room(name, per_wall=per_wall)
+    gp, _ = _grouped_and_generic(ops, r, 343., 1 / 2000)
+    tp = ops.tile_plan(gp, jt, min_merge=0)
+    P, D, B, T = gp.P, gp.D, gp.B, 1300
+    st = _random_state(ops, P, D, B, T, seed=jt)
+    st.gather(gp)
+    ref = st.gsum[:gp.n_groups].clone()
+    st.gsum.fill_(-1.0)
+    gp.tiles = tp
+    st.gather(gp)
+    torch.cuda.synchronize()
+    assert torch.equal(st.gsum[:gp.n_groups], ref)
+
+
+@pytest.mark.parametrize("jt", [4, 8])
+def test_tiled_exchange_matches_oracle(ops, jt):
+    """Whole exchange on tiles: equal to the untiled grouped plan and
+    within the exchange tolerance of the oracle's FP64 loop."""
+    import torch
+    r = _room("2.5x1.5x1", seed=5)
+    r.init_source_energy(np.array([1.1, .7, .4]))
+    T, c, dt, K = 400, 343., 1 / 2000, 8
+    gp, _ = _grouped_and_generic(ops, r, c, dt)
+    P, D, B = r.energy_init_source.shape
+    outs = []
+    for tiles in (None, ops.tile_plan(gp, jt, min_merge=0)):
+        gp.tiles = tiles
+        st = ops.ExchangeState(P, D, B, T)
+        st.init(r.energy_init_source, r.distance_patches_to_source, c, dt)
+        for _ in range(K):
+            st.step(gp)
+        st.check()
+        outs.append(st.finalize())
+    assert torch.equal(outs[0], outs[1])
+    ref = orc.exchange(T, r.energy_init_source, r.distance_patches_to_source,
+                       r.pair_distances(), r.fft, r.p2o, c, dt, K,
+                       r.visible_patches, nthreads=8)
+    assert rel_err_within_db(_np(outs[1]), ref) <= EXCHANGE_RTOL
+
+
+def test_class_path_uses_tiles(ops):
+    """DirectionalRadiosityFast builds tiles for its grouped plan when they
+    merge entries, and its histograms do not depend on them."""
+    import torch
+    from sparrowpy_amd import DirectionalRadiosityFast
+    from sparrowpy_amd import brdf as B
+    from sparrowpy_amd import samplings
+    from sparrowpy_amd.coordinates import Coordinates, FrequencyData
+    freqs = np.array([500., 1000.])
+    dirs = samplings.hemisphere(samplings.sph_gaussian(sh_order=3))
+    br = B.create_from_scattering(
+        dirs, dirs, FrequencyData(np.array([.3, .6]), freqs),
+        FrequencyData(np.array([.1, .2]), freqs))
+    hists = []
+    for rows in (0, 4):
+        r = DirectionalRadiosityFast.from_polygon(
+            testing.shoebox_room_stub(3, 2.5, 2), .5)
+        r.set_wall_brdf(np.arange(6), br, dirs, dirs)
+        r.set_air_attenuation(FrequencyData(np.array([1e-3, 2e-3]), freqs))
+        r.TILE_ROWS, r.TILE_MIN_MERGE = rows, 0.
+        r.bake_geometry()
+        r.init_source_energy(Coordinates(1.2, .9, .7))
+        r.calculate_energy_exchange(343., 1 / 2000, .2, 6)
+        plan = r._exchange_plan(343., 1 / 2000)
+        assert isinstance(plan, ops.GroupedPlan)
+        assert (plan.tiles is not None) == bool(rows)
+        hists.append(r.collect_energy_receiver_mono(
+            Coordinates(1.5, 1.2, 1.)).time)
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(hists[0], hists[1])
