@@ -11,8 +11,9 @@ per order covers the whole class and results are bit-for-bit those of the
 per-instance runs (same entries, same per-row order).  The K orders of a
 class are captured once into a HIP graph and replayed; per class the plan
 kind (grouped or diffuse + specular) is picked by timing one order of each
-candidate on the class's largest scene (``autotune``).  Classes larger than the memory budget are
-split into chunks.
+candidate on the class's largest scene (``autotune``); a grouped class
+gathers on destination tiles of ``tile_rows`` rows (``ops.tile_plan``).
+Classes larger than the memory budget are split into chunks.
 """
 from __future__ import annotations
 
@@ -94,8 +95,10 @@ class BatchedExchange:
     """
 
     def __init__(self, items, speed_of_sound, time_resolution, duration,
-                 max_order, autotune=True, memory_budget=None, graphs=True):
+                 max_order, autotune=True, memory_budget=None, graphs=True,
+                 tile_rows=4):
         self.items = list(items)
+        self.tile_rows = int(tile_rows)
         self.c, self.dt = float(speed_of_sound), float(time_resolution)
         self.T = int(duration / time_resolution)
         self.K = int(max_order)
@@ -187,6 +190,9 @@ class BatchedExchange:
             [[0], np.cumsum([self.items[x].radiosity.n_patches
                              for x in xs])]).astype(np.int64)
         plan = _concat_plans([plans[x] for x in xs], offsets[:-1])
+        if (self.tile_rows and isinstance(plan, ops.GroupedPlan) and
+                not plan.diffuse_specular):
+            plan.tiles = ops.tile_plan(plan, self.tile_rows)
         e0d, d0 = [], []
         for x, s in chunk:
             r = self.items[x].radiosity

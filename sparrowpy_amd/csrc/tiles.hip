@@ -182,39 +182,46 @@ __global__ __launch_bounds__(256) void tile_flag_kernel(
 // flight while chunk q is summed.  A slot without an entry has delay
 // kLagMask (a descriptor of size 0: loads give 0) and weight 0.
 // Requires T < 2^29.
-template <int JT>
+template <int JW>
 struct TileMeta {
   int32_t row;
-  int32_t lag[JT];
-  double w[JT];
+  int32_t lag[JW];
+  double w[JW];
 };
 
-template <int JT>
-__device__ __forceinline__ TileMeta<JT> load_tile_meta(
-    int64_t m0, int64_t m_end, const int32_t *__restrict__ mrow,
-    const int32_t *__restrict__ mlag, const double *__restrict__ w) {
+// Block of merged entries m0 .. m0+63 (one per lane), slots s0 .. s0+JW-1
+// of JT.
+template <int JW>
+__device__ __forceinline__ TileMeta<JW> load_tile_meta(
+    int64_t m0, int64_t m_end, int JT, int s0,
+    const int32_t *__restrict__ mrow, const int32_t *__restrict__ mlag,
+    const double *__restrict__ w) {
   const int64_t m = m0 + (int64_t)(threadIdx.x & 63);
   const bool in = m < m_end;
   const int64_t mc = in ? m : m_end - 1;
-  TileMeta<JT> t;
+  TileMeta<JW> t;
   t.row = mrow[mc];
 #pragma unroll
-  for (int k = 0; k < JT; ++k) {
-    t.lag[k] = in ? mlag[mc * JT + k] : kLagMask;
-    t.w[k] = in ? w[mc * JT + k] : 0.0;
+  for (int k = 0; k < JW; ++k) {
+    t.lag[k] = in ? mlag[mc * JT + s0 + k] : kLagMask;
+    t.w[k] = in ? w[mc * JT + s0 + k] : 0.0;
   }
   return t;
 }
 
-template <int R, int BS, int C, int JT, int MB>
-__global__ __launch_bounds__(BS) void tile_gather_kernel(
+// A workgroup takes one (tile, band, block of 256*R bins) with JT/JW
+// groups of 4 waves: group h sums slots h*JW .. h*JW+JW-1 over the same
+// bins, so the groups' gathers of one source row meet in the CU's cache.
+template <int R, int C, int JT, int JW, int MB>
+__global__ __launch_bounds__(256 * (JT / JW)) void tile_gather_kernel(
     int64_t P, int B, int T, int n_tblk, int64_t nblocks, int64_t M,
     const int64_t *__restrict__ tptr, const int64_t *__restrict__ grp_ptr,
     const int32_t *__restrict__ mrow, const int32_t *__restrict__ mlag,
     const double *__restrict__ mw, const float *__restrict__ prev,
     float *__restrict__ gsum) {
   static_assert(MB <= 64 && MB % (2 * C) == 0, "chunks pair up in a block");
-  constexpr int NQ = MB / C;
+  static_assert(JT % JW == 0, "slot groups split the tile");
+  constexpr int NQ = MB / C, BS = 256;
   int64_t tile;
   int b, tb;
   dest_major_tile(xcd_tile(blockIdx.x, nblocks), nblocks / (B * n_tblk),
@@ -223,24 +230,25 @@ __global__ __launch_bounds__(BS) void tile_gather_kernel(
   const int t_end = min(T, t0 + BS * R);
   const int64_t m_begin = tptr[tile], m_end = tptr[tile + 1];
   if (m_end <= m_begin) return;
+  const int s0 = (int)(threadIdx.x / BS) * JW;  // first slot of this group
   const double *w = mw + (int64_t)b * M * JT;
   const uint32_t row_bytes = (uint32_t)T * 4u;
-  const int tl = t0 + (int)threadIdx.x;
-  int64_t g[JT];
+  const int tl = t0 + (int)(threadIdx.x % BS);
+  int64_t g[JW];
 #pragma unroll
-  for (int k = 0; k < JT; ++k) {
-    const int64_t j = tile * JT + k;
+  for (int k = 0; k < JW; ++k) {
+    const int64_t j = tile * JT + s0 + k;
     g[k] = j < P ? grp_ptr[j] : 0;
   }
 
-  auto fetch = [&](const TileMeta<JT> &m, int q, float (&v)[C][JT][R]) {
+  auto fetch = [&](const TileMeta<JW> &m, int q, float (&v)[C][JW][R]) {
 #pragma unroll
     for (int c = 0; c < C; ++c) {
       const int idx = q * C + c;
       const int32_t rk = __builtin_amdgcn_readlane(m.row, idx);
       const float *src = prev + ((int64_t)rk * B + b) * T;
 #pragma unroll
-      for (int k = 0; k < JT; ++k) {
+      for (int k = 0; k < JW; ++k) {
         const int n = min(__builtin_amdgcn_readlane(m.lag[k], idx) & kLagMask,
                           T);
         const __amdgpu_buffer_rsrc_t rs =
@@ -252,25 +260,25 @@ __global__ __launch_bounds__(BS) void tile_gather_kernel(
     }
   };
 
-  double S[JT][R];
+  double S[JW][R];
 #pragma unroll
-  for (int k = 0; k < JT; ++k)
+  for (int k = 0; k < JW; ++k)
 #pragma unroll
     for (int r = 0; r < R; ++r) S[k][r] = 0.0;
-  auto consume = [&](const TileMeta<JT> &m, int q,
-                     const float (&v)[C][JT][R]) {
+  auto consume = [&](const TileMeta<JW> &m, int q,
+                     const float (&v)[C][JW][R]) {
 #pragma unroll
     for (int c = 0; c < C; ++c) {
       const int idx = q * C + c;
 #pragma unroll
-      for (int k = 0; k < JT; ++k) {
+      for (int k = 0; k < JW; ++k) {
         const double wk = readlane_f64(m.w[k], idx);
 #pragma unroll
         for (int r = 0; r < R; ++r)
           S[k][r] = fma(wk, (double)v[c][k][r], S[k][r]);
       }
 #pragma unroll
-      for (int k = 0; k < JT; ++k) {
+      for (int k = 0; k < JW; ++k) {
         if (__builtin_amdgcn_readlane(m.lag[k], idx) & kLastInGroup) {
           const __amdgpu_buffer_rsrc_t os =
               make_rsrc(gsum + (g[k] * B + b) * (int64_t)T, row_bytes);
@@ -285,10 +293,11 @@ __global__ __launch_bounds__(BS) void tile_gather_kernel(
     }
   };
 
-  TileMeta<JT> cur = load_tile_meta<JT>(m_begin, m_end, mrow, mlag, w);
-  TileMeta<JT> next =
-      load_tile_meta<JT>(m_begin + MB, m_end, mrow, mlag, w);
-  float va[C][JT][R], vb[C][JT][R];
+  TileMeta<JW> cur =
+      load_tile_meta<JW>(m_begin, m_end, JT, s0, mrow, mlag, w);
+  TileMeta<JW> next =
+      load_tile_meta<JW>(m_begin + MB, m_end, JT, s0, mrow, mlag, w);
+  float va[C][JW][R], vb[C][JW][R];
   fetch(cur, 0, va);
   for (int64_t blk = m_begin; blk < m_end; blk += MB) {
 #pragma unroll
@@ -302,8 +311,34 @@ __global__ __launch_bounds__(BS) void tile_gather_kernel(
       consume(cur, q + 1, vb);
     }
     cur = next;
-    next = load_tile_meta<JT>(blk + 2 * MB, m_end, mrow, mlag, w);
+    next = load_tile_meta<JW>(blk + 2 * MB, m_end, JT, s0, mrow, mlag, w);
   }
+}
+
+// Launch shapes, measured on the street canyon (T = 6,000): JT = 4 (5.2 M
+// merged entries) with R = 4 bins per lane and one merged entry per chunk
+// (118 VGPRs, 4 waves per SIMD) takes 166 ms; R = 2 with two entries per
+// chunk 211 ms; R = 8, 184 ms.  JT = 8 (3.3 M merged entries, 38 % of the
+// slots empty) in two groups of 4 waves takes 213 ms: its lower traffic does
+// not pay for the empty slots' issue slots.
+template <int R, int C, int JT, int JW, int MB>
+int launch_tile_shape(int64_t P, int64_t B, int64_t T, int64_t M,
+                      const int64_t *tptr, const int64_t *grp_ptr,
+                      const int32_t *mrow, const int32_t *mlag,
+                      const double *mw, const float *prev, float *gsum,
+                      hipStream_t s) {
+  constexpr int BS = 256;
+  const int64_t n_tiles = ceil_div(P, JT);
+  const int64_t nt = ceil_div(T, (int64_t)BS * R);
+  const int64_t nb = n_tiles * B * nt;
+  SPR_REQUIRE(nb < (int64_t)0x7fffffff, "exchange grid too large");
+  SPR_REQUIRE(T < ((int64_t)1 << 29), "histogram too long (T >= 2^29)");
+  hipLaunchKernelGGL((tile_gather_kernel<R, C, JT, JW, MB>),
+                     dim3((unsigned)nb), dim3(BS * (JT / JW)), 0, s, P,
+                     (int)B, (int)T, (int)nt, nb, M, tptr, grp_ptr, mrow,
+                     mlag, mw, prev, gsum);
+  SPR_LAUNCH_CHECK();
+  return SPR_OK;
 }
 
 int launch_tile_gather(int64_t P, int JT, int64_t B, int64_t T, int64_t M,
@@ -311,24 +346,11 @@ int launch_tile_gather(int64_t P, int JT, int64_t B, int64_t T, int64_t M,
                        const int32_t *mrow, const int32_t *mlag,
                        const double *mw, const float *prev, float *gsum,
                        hipStream_t s) {
-  constexpr int BS = 256, R = 2;
-  const int64_t n_tiles = ceil_div(P, JT);
-  const int64_t nt = ceil_div(T, (int64_t)BS * R);
-  const int64_t nb = n_tiles * B * nt;
-  SPR_REQUIRE(nb < (int64_t)0x7fffffff, "exchange grid too large");
-  SPR_REQUIRE(T < ((int64_t)1 << 29), "histogram too long (T >= 2^29)");
   if (JT == 4)
-    hipLaunchKernelGGL((tile_gather_kernel<R, BS, 2, 4, 64>),
-                       dim3((unsigned)nb), dim3(BS), 0, s, P, (int)B, (int)T,
-                       (int)nt, nb, M, tptr, grp_ptr, mrow, mlag, mw, prev,
-                       gsum);
-  else
-    hipLaunchKernelGGL((tile_gather_kernel<R, BS, 1, 8, 32>),
-                       dim3((unsigned)nb), dim3(BS), 0, s, P, (int)B, (int)T,
-                       (int)nt, nb, M, tptr, grp_ptr, mrow, mlag, mw, prev,
-                       gsum);
-  SPR_LAUNCH_CHECK();
-  return SPR_OK;
+    return launch_tile_shape<4, 1, 4, 4, 64>(P, B, T, M, tptr, grp_ptr, mrow,
+                                             mlag, mw, prev, gsum, s);
+  return launch_tile_shape<4, 1, 8, 4, 64>(P, B, T, M, tptr, grp_ptr, mrow,
+                                           mlag, mw, prev, gsum, s);
 }
 
 unsigned grid_of(int64_t n) { return (unsigned)ceil_div(n > 0 ? n : 1, 256); }
@@ -347,8 +369,8 @@ extern "C" int spr_plan_tile_order(int64_t P, int64_t JT, int64_t Vd,
                                    int64_t *first, int64_t *tptr,
                                    void *stream) {
   clear_error();
-  SPR_REQUIRE(P > 0 && Vd >= 0 && G >= 0 && (JT == 4 || JT == 8),
-              "spr_plan_tile_order: bad sizes (JT must be 4 or 8)");
+  SPR_REQUIRE(P > 0 && Vd >= 0 && G >= 0 && JT >= 2 && JT <= 8,
+              "spr_plan_tile_order: bad sizes (JT must be 2..8)");
   SPR_REQUIRE(Vd < ((int64_t)1 << 40), "spr_plan_tile_order: plan too large");
   SPR_REQUIRE(row_ptr && tptr && first, "spr_plan_tile_order: null pointer");
   SPR_REQUIRE(Vd == 0 || (grp_start && grp_key && rowid && skey && sorted &&
@@ -384,9 +406,9 @@ extern "C" int spr_plan_tile_fill(int64_t P, int64_t JT, int64_t B,
                                   const int64_t *first, int32_t *mrow,
                                   int32_t *mlag, double *mw, void *stream) {
   clear_error();
-  SPR_REQUIRE(P > 0 && B > 0 && Vd >= 0 && M >= 0 && M <= Vd &&
-                  (JT == 4 || JT == 8),
-              "spr_plan_tile_fill: bad sizes (JT must be 4 or 8)");
+  SPR_REQUIRE(P > 0 && B > 0 && Vd >= 0 && M >= 0 && M <= Vd && JT >= 2 &&
+                  JT <= 8,
+              "spr_plan_tile_fill: bad sizes (JT must be 2..8)");
   if (M == 0) return SPR_OK;
   SPR_REQUIRE(row_ptr && rowid && lag && scal && sorted && payload && first &&
                   mrow && mlag && mw,
