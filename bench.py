@@ -389,6 +389,7 @@ def main(argv=None):
     n_ev = 3 if two_pass else 2
     ev = [[torch.cuda.Event(enable_timing=True) for _ in range(n_ev)]
           for _ in range(args.steps)]
+    n_pair = 0
     torch.cuda.synchronize()
     wall0 = time.perf_counter()
     for k in range(args.steps):
@@ -396,6 +397,7 @@ def main(argv=None):
         if two_pass:
             state.gather(plan)
             ev[k][1].record(stream)
+            n_pair += state.pending     # this spread adds two orders
             state.spread(plan)
             state.prev, state.cur = state.cur, state.prev
         else:
@@ -433,8 +435,11 @@ def main(argv=None):
         # one write of the per-group sums
         n_w = 2 if kind == "ds" else 1
         gather_bytes = 4 * U + Vd * (8 + 8 * B * n_w) + 4 * G * B * T
-        # spread: read the group sums, write cur, read+write hi and lo
-        spread_bytes = 4 * G * B * T + 20 * n_state
+        # spread: read the group sums, write cur; every other order also
+        # reads the previous order and reads + writes hi and lo (20 bytes
+        # per element)
+        spread_bytes = (4 * G * B * T + 4 * n_state +
+                        20 * n_state * n_pair / args.steps)
         gather_name = ("tile_gather_kernel" if tiles is not None else
                        "group_gather_kernel")
         kernels = {

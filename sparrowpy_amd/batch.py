@@ -225,10 +225,13 @@ class BatchedExchange:
                     st.step(plan)
         torch.cuda.current_stream().wait_stream(side)
         # capture recorded the launches without running them: reset the
-        # state to order 0 and replay
+        # state to order 0 and replay (after which an odd K leaves its last
+        # order out of the running total, as the captured steps did)
+        pending = st.pending
         st.prev, st.cur = a, b
         st.init(run["e0d"], run["d0"], self.c, self.dt)
         graph.replay()
+        st.pending = pending
         if self.K % 2:
             st.prev, st.cur = b, a
         run["graph"] = graph

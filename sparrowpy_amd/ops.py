@@ -591,6 +591,9 @@ class ExchangeState:
         self.lo = torch.empty(shape, dtype=torch.float32, device=dev)
         self.err = torch.zeros(1, dtype=torch.int32, device=dev)
         self.gsum = None  # per-group sums of the grouped plan (workspace)
+        # grouped steps add orders to the running total in pairs; True while
+        # the last order (in prev after the swap) is not yet in it
+        self.pending = False
 
     def init(self, energy_0_directivity, distance_0, speed_of_sound,
              time_resolution):
@@ -599,6 +602,7 @@ class ExchangeState:
         d0 = _f64(distance_0)
         P, D, B, T = self.shape
         self.err.zero_()
+        self.pending = False
         call("spr_exchange_init", P, D, B, T, ptr(e0d), ptr(d0),
              float(speed_of_sound), float(time_resolution), ptr(self.prev),
              ptr(self.hi), ptr(self.lo), ptr(self.err), stream_ptr())
@@ -611,6 +615,7 @@ class ExchangeState:
             self.gather(plan)
             self.spread(plan)
         else:
+            self.flush()
             call("spr_exchange_step", P, D, B, T, ptr(plan.row_ptr),
                  ptr(plan.src), ptr(plan.lag), ptr(plan.dir), ptr(plan.coef),
                  ptr(self.prev), ptr(self.cur), ptr(self.hi), ptr(self.lo),
@@ -647,15 +652,33 @@ class ExchangeState:
              ptr(self._gsum(plan)), stream_ptr())
 
     def spread(self, plan):
-        """Grouped plan, pass 2 of an order (directions and totals); does
-        not swap prev and cur."""
+        """Grouped plan, pass 2 of an order (directions; the running total
+        every other order, for this order and the previous one); does not
+        swap prev and cur."""
         P, D, B, T = self.shape
-        call("spr_exchange_spread_grouped", P, D, B, T, ptr(plan.grp_ptr),
-             ptr(plan.grp_key), ptr(plan.rtab), ptr(self._gsum(plan)),
-             ptr(self.cur), ptr(self.hi), ptr(self.lo), stream_ptr())
+        if self.pending:
+            call("spr_exchange_spread_grouped_pair", P, D, B, T,
+                 ptr(plan.grp_ptr), ptr(plan.grp_key), ptr(plan.rtab),
+                 ptr(self._gsum(plan)), ptr(self.prev), ptr(self.cur),
+                 ptr(self.hi), ptr(self.lo), stream_ptr())
+        else:
+            call("spr_exchange_spread_grouped", P, D, B, T,
+                 ptr(plan.grp_ptr), ptr(plan.grp_key), ptr(plan.rtab),
+                 ptr(self._gsum(plan)), ptr(self.cur), None, None,
+                 stream_ptr())
+        self.pending = not self.pending
+
+    def flush(self):
+        """Add an order still left out of the running total (the last
+        one, in prev after the swap)."""
+        if self.pending:
+            call("spr_exchange_total_add", self.hi.numel(), ptr(self.prev),
+                 ptr(self.hi), ptr(self.lo), stream_ptr())
+            self.pending = False
 
     def finalize(self, out=None):
         """float32 total (hi + lo); ``out`` may be ``self.hi``."""
+        self.flush()
         out = torch.empty_like(self.hi) if out is None else out
         call("spr_exchange_finalize", self.hi.numel(), ptr(self.hi),
              ptr(self.lo), ptr(out), stream_ptr())

@@ -177,3 +177,32 @@ def test_class_path_uses_tiles(ops):
             Coordinates(1.5, 1.2, 1.)).time)
     torch.cuda.synchronize()
     np.testing.assert_array_equal(hists[0], hists[1])
+
+
+@pytest.mark.parametrize("K", [1, 2, 3, 6, 7])
+def test_totals_added_in_pairs(ops, K):
+    """Grouped steps add orders to the running total two at a time (an odd
+    last order is added by finalize): totals match the generic plan's,
+    which adds every order, for even and odd order counts."""
+    import torch
+    r = _room("2.5x1.5x1", seed=7)
+    r.init_source_energy(np.array([.9, .6, .5]))
+    T, c, dt = 300, 343., 1 / 2000
+    gp, gen = _grouped_and_generic(ops, r, c, dt)
+    gp.tiles = ops.tile_plan(gp, 4, min_merge=0)
+    P, D, B = r.energy_init_source.shape
+    outs = []
+    for pl in (gen, gp):
+        st = ops.ExchangeState(P, D, B, T)
+        st.init(r.energy_init_source, r.distance_patches_to_source, c, dt)
+        for k in range(K):
+            st.step(pl)
+            assert st.pending == (isinstance(pl, ops.GroupedPlan) and
+                                  k % 2 == 0)
+        outs.append(st.finalize())
+        assert not st.pending
+        # a second finalize sees the same total
+        assert torch.equal(st.finalize(), outs[-1])
+    a, b = _np(outs[0]).astype(np.float64), _np(outs[1]).astype(np.float64)
+    assert np.abs(a - b).max() <= 1e-6 * np.abs(a).max()
+    assert rel_err_within_db(b, a) <= 1e-6
