@@ -26,12 +26,20 @@ __global__ void blocker_kernel(int64_t NS, const double *__restrict__ normals,
   out[k] = r;
 }
 
-// Thread per pair (i, j) of a 16x16 tile; every lane walks the blockers in
-// order (a uniform loop, so records arrive through scalar loads) until the
-// pair is blocked; the wave leaves the loop once all its pairs are decided.
+// Thread per pair (i, j) of a 16x16 tile.  The workgroup first culls the
+// blockers for the whole tile (may_block_tile over the boxes of its 16 row
+// and 16 column patch centres) into a candidate list in LDS, 256 blockers
+// at a time; every lane then walks the candidates (a uniform loop, so
+// records arrive through scalar loads) until its pair is blocked.  The cull
+// only drops blockers that cannot block any pair of the tile, so the flags
+// are those of the full scan (canyon, 5,122 patches as blockers: ~1,500
+// candidates per tile, 105 ms instead of 277 ms).
 __global__ __launch_bounds__(256) void p2p_visibility_kernel(
     int64_t P, const double *__restrict__ centers, int64_t NS,
     const BlockerRec *__restrict__ recs, uint8_t *__restrict__ vis) {
+  __shared__ int32_t cand[256];
+  __shared__ int32_t wave_count[4];
+  __shared__ double box[12];
   const int64_t ti = blockIdx.y, tj = blockIdx.x;
   const int64_t i = ti * 16 + (threadIdx.x >> 4);
   const int64_t j = tj * 16 + (threadIdx.x & 15);
@@ -46,9 +54,45 @@ __global__ __launch_bounds__(256) void p2p_visibility_kernel(
     a = load3(centers + 3 * i);
     b = load3(centers + 3 * j);
   }
-  for (int64_t k = 0; k < NS; ++k) {
-    if (!__any(alive)) break;
-    if (alive) alive = basic_visibility(a, b, recs[k]);
+  if (threadIdx.x < 2) {
+    // bounding box of the tile's row (i) or column (j) patch centres
+    const int64_t t = threadIdx.x == 0 ? ti : tj;
+    d3 lo{INFINITY, INFINITY, INFINITY}, hi{-INFINITY, -INFINITY, -INFINITY};
+    for (int q = 0; q < 16; ++q) {
+      const int64_t k = t * 16 + q;
+      if (k >= P) break;
+      const d3 c = load3(centers + 3 * k);
+      lo = d3{fmin(lo.x, c.x), fmin(lo.y, c.y), fmin(lo.z, c.z)};
+      hi = d3{fmax(hi.x, c.x), fmax(hi.y, c.y), fmax(hi.z, c.z)};
+    }
+    double *o = box + 6 * threadIdx.x;
+    o[0] = lo.x, o[1] = lo.y, o[2] = lo.z, o[3] = hi.x, o[4] = hi.y,
+    o[5] = hi.z;
+  }
+  __syncthreads();
+  const d3 ilo{box[0], box[1], box[2]}, ihi{box[3], box[4], box[5]};
+  const d3 jlo{box[6], box[7], box[8]}, jhi{box[9], box[10], box[11]};
+  const int wave = (int)threadIdx.x >> 6, lane = (int)threadIdx.x & 63;
+  for (int64_t k0 = 0; k0 < NS; k0 += 256) {
+    const int64_t kk = k0 + (int64_t)threadIdx.x;
+    const bool keep = kk < NS && may_block_tile(ilo, ihi, jlo, jhi, recs[kk]);
+    const uint64_t m = __ballot(keep);
+    if (lane == 0) wave_count[wave] = __popcll(m);
+    __syncthreads();
+    int base = 0, n_cand = 0;
+    for (int w = 0; w < 4; ++w) {
+      base += w < wave ? wave_count[w] : 0;
+      n_cand += wave_count[w];
+    }
+    if (keep)
+      cand[base + __popcll(m & ((1ull << lane) - 1ull))] = (int32_t)kk;
+    __syncthreads();
+    for (int c = 0; c < n_cand; ++c) {
+      if (!__any(alive)) break;
+      const int k = __builtin_amdgcn_readfirstlane(cand[c]);
+      if (alive) alive = basic_visibility(a, b, recs[k]);
+    }
+    if (!__syncthreads_or(alive)) break;
   }
   if (valid) vis[i * P + j] = (uint8_t)alive;
 }

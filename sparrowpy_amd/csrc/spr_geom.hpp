@@ -4,7 +4,7 @@
 // The predicate reproduces geometry._basic_visibility /
 // _point_in_polygon / _project_to_plane / _rotation_matrix
 // (sparrowpy/geometry.py:498-686, 841-909) operation for operation, so the
-// boolean results are the reference's bits.  Two things differ in
+// boolean results are the reference's bits.  Three things differ in
 // organisation only, never in result:
 //  * Everything that depends on the blocking polygon alone (rotation rows,
 //    rotated corners, edge normals and lengths, the rotated bounding box)
@@ -17,6 +17,8 @@
 //    "segment crosses the polygon" test evaluates the cheap
 //    (pt - a).(pt - b) < 0 condition before the winding test (both are pure
 //    and are combined with AND).
+//  * Blockers that cannot block any segment between two boxes of points
+//    (may_block_tile) are not evaluated for those segments.
 #pragma once
 
 #include "spr_common.hpp"
@@ -146,6 +148,67 @@ __device__ inline bool point_in_blocker(d3 p, const BlockerRec &r) {
     }
   }
   return count != 0;
+}
+
+// Culling that keeps basic_visibility's result.  The predicate can only
+// return "blocked" for a segment a-b if a point of it (an endpoint, or the
+// crossing point, which lies on the segment up to rounding) passes
+// point_in_blocker: within kEta of the plane, at rotated x <= box[1] +
+// kBoxMargin and rotated y within box[2..3] +- kBoxMargin.  Such points lie
+// on the part of the segment within kCullPlane of the plane, and the
+// rotated x and y are linear along it, so a blocker whose accepted x / y
+// range that part misses by more than kCullBox leaves the pair visible --
+// as does a blocker whose plane the segment never comes within kCullPlane
+// of.  The margins exceed the thresholds by far more than any rounding of
+// points on the segment.
+constexpr double kCullPlane = 1e-3;  // >> kEta
+constexpr double kCullBox = 2e-3;    // > kBoxMargin
+
+// Interval bounds of the linear form c.p over the axis-aligned box [lo, hi].
+__device__ inline double form_min(d3 c, d3 lo, d3 hi) {
+  return (fmin(c.x * lo.x, c.x * hi.x) + fmin(c.y * lo.y, c.y * hi.y)) +
+         fmin(c.z * lo.z, c.z * hi.z);
+}
+__device__ inline double form_max(d3 c, d3 lo, d3 hi) {
+  return (fmax(c.x * lo.x, c.x * hi.x) + fmax(c.y * lo.y, c.y * hi.y)) +
+         fmax(c.z * lo.z, c.z * hi.z);
+}
+
+// For every segment from a point of box I to a point of box J.  If one box
+// lies within kCullPlane of the plane and the other beyond it (a patch's
+// own wall seen from off the wall), the part of each segment within
+// kCullPlane of the plane stays within tau * L of the first box, with
+// tau = 2 kCullPlane / (gap - kCullPlane) and L the longest segment, so
+// only that neighbourhood of the box is tested; otherwise the whole union
+// box is.
+__device__ inline bool may_block_tile(d3 ilo, d3 ihi, d3 jlo, d3 jhi,
+                                      const BlockerRec &r) {
+  const d3 n = load3(r.n), m0 = load3(r.m0), m1 = load3(r.m1);
+  const double np0 = dot(n, load3(r.p0));
+  const double di0 = form_min(n, ilo, ihi) - np0;
+  const double di1 = form_max(n, ilo, ihi) - np0;
+  const double dj0 = form_min(n, jlo, jhi) - np0;
+  const double dj1 = form_max(n, jlo, jhi) - np0;
+  if ((di0 > kCullPlane && dj0 > kCullPlane) ||
+      (di1 < -kCullPlane && dj1 < -kCullPlane))
+    return false;
+  const d3 ulo{fmin(ilo.x, jlo.x), fmin(ilo.y, jlo.y), fmin(ilo.z, jlo.z)};
+  const d3 uhi{fmax(ihi.x, jhi.x), fmax(ihi.y, jhi.y), fmax(ihi.z, jhi.z)};
+  d3 lo = ulo, hi = uhi;
+  double grow = 0.0;
+  const bool i_on = di0 >= -kCullPlane && di1 <= kCullPlane;
+  const bool j_on = dj0 >= -kCullPlane && dj1 <= kCullPlane;
+  const double gap_j = fmax(dj0, -dj1), gap_i = fmax(di0, -di1);
+  if (i_on && !j_on && gap_j > 2 * kCullPlane) {
+    lo = ilo, hi = ihi;
+    grow = 2 * kCullPlane / (gap_j - kCullPlane) * norm(uhi - ulo);
+  } else if (j_on && !i_on && gap_i > 2 * kCullPlane) {
+    lo = jlo, hi = jhi;
+    grow = 2 * kCullPlane / (gap_i - kCullPlane) * norm(uhi - ulo);
+  }
+  if (form_min(m0, lo, hi) - grow > r.box[1] + kCullBox) return false;
+  return !(form_max(m1, lo, hi) + grow < r.box[2] - kCullBox ||
+           form_min(m1, lo, hi) - grow > r.box[3] + kCullBox);
 }
 
 // geometry.py:841-909 _basic_visibility(vis_point=a, eval_point=b).

@@ -40,6 +40,18 @@ def _scene_geometry(walls, ps):
                 area=geometry.calculate_area(pts), ids=ids, wp=wp, wn=wn)
 
 
+def _rotated(walls, ps, seed):
+    """Patches of the walls, then a seeded rotation of the whole scene
+    (planes off the axes)."""
+    g = _scene_geometry(walls, ps)
+    rng = np.random.default_rng(seed)
+    q, _ = np.linalg.qr(rng.normal(size=(3, 3)))
+    for k in ("pts", "nrm", "wp", "wn"):
+        g[k] = np.ascontiguousarray(g[k] @ q.T)
+    g["ctr"] = geometry.calculate_center(g["pts"])
+    return g
+
+
 GEOMS = {
     "cube-0.5": lambda: _scene_geometry(testing.shoebox_room_stub(1, 1, 1),
                                         .5),
@@ -49,6 +61,8 @@ GEOMS = {
         seed=5, length=12., width=6., height=5., n_recesses=3).walls, 1.),
     "canyon-mid": lambda: _scene_geometry(scenes.street_canyon(
         seed=7, length=24., width=8., height=6., n_recesses=4).walls, 1.),
+    "canyon-rotated": lambda: _rotated(scenes.street_canyon(
+        seed=5, length=12., width=6., height=5., n_recesses=3).walls, 1., 11),
 }
 
 
