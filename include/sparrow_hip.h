@@ -275,6 +275,17 @@ int spr_exchange_spread_grouped_pair(int64_t P, int64_t D, int64_t B,
                                      float *tot_hi, float *tot_lo,
                                      void *stream);
 
+/* Spread pass of the diffuse + specular plan: the same cur (and total) as
+ * spr_exchange_spread_grouped on that plan, bit for bit, formed as
+ * cur[j,d] = gsum[diffuse group of j] + gsum[group of j with key d]
+ * instead of D multiply-adds per group.  prev, tot_hi, tot_lo as in the
+ * two grouped spreads (all NULL: histogram only; prev NULL: add cur to the
+ * total; else add prev + cur). */
+int spr_exchange_spread_ds(int64_t P, int64_t D, int64_t B, int64_t T,
+                           const int64_t *grp_ptr, const int32_t *grp_key,
+                           const float *gsum, const float *prev, float *cur,
+                           float *tot_hi, float *tot_lo, void *stream);
+
 /* tot_hi + tot_lo += x[n] (compensated, float64 arithmetic): adds an order
  * left out of the total by the pairing above. */
 int spr_exchange_total_add(int64_t n, const float *x, float *tot_hi,

@@ -89,6 +89,8 @@ SYMBOLS = {
                                                 _vp, _vp, _vp, _vp, _vp, _vp,
                                                 _vp, _vp]),
     "spr_exchange_total_add": (_int, [_i64, _vp, _vp, _vp, _vp]),
+    "spr_exchange_spread_ds": (_int, [_i64, _i64, _i64, _i64, _vp, _vp, _vp,
+                                      _vp, _vp, _vp, _vp, _vp]),
     "spr_exchange_step_grouped": (_int, [_i64, _i64, _i64, _i64, _i64, _vp,
                                          _vp, _vp, _vp, _vp, _vp, _vp, _vp,
                                          _vp, _vp, _vp, _vp, _vp]),

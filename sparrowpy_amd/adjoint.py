@@ -138,9 +138,9 @@ class ExchangeAdjoint:
              ptr(p.row_ptr), ptr(p.grp_ptr), ptr(p.rowid), ptr(p.lag),
              ptr(p.scal), ptr(p.dscal), ptr(state_prev), ptr(gsum),
              stream_ptr())
-        call("spr_exchange_spread_grouped", self.P, self.D, self.B, self.T,
-             ptr(p.grp_ptr), ptr(p.grp_key), ptr(p.rtab), ptr(gsum),
-             ptr(out), None, None, stream_ptr())
+        call("spr_exchange_spread_ds", self.P, self.D, self.B, self.T,
+             ptr(p.grp_ptr), ptr(p.grp_key), ptr(gsum), None, ptr(out),
+             None, None, stream_ptr())
 
     def forward(self, A, C):
         """Receiver histograms (float32 [R, B, T]) for tables (A, C);

@@ -59,6 +59,11 @@ def _concat_plans(plans, offsets):
         goff = np.concatenate([[0], np.cumsum(gs)]).astype(np.int64)
         ks = [p.rtab.shape[0] for p in plans]
         koff = np.concatenate([[0], np.cumsum(ks)]).astype(np.int64)
+        if p0.diffuse_specular:
+            # the diffuse + specular table depends on (D, B) only: shared,
+            # keys kept (the diffuse + specular spread reads them as
+            # directions)
+            koff[:] = 0
         grp = [p.grp_ptr[:-1] + int(goff[n]) for n, p in enumerate(plans)]
         grp.append(torch.tensor([goff[-1]], dtype=torch.int64, device=dev))
         starts = [p.grp_start[:-1] + int(voff[n]) for n, p in
@@ -71,7 +76,7 @@ def _concat_plans(plans, offsets):
             _cat([p.rowid + int(offsets[n]) * D for n, p in
                   enumerate(plans)]),
             lag, _cat([p.scal for p in plans], 1),
-            _cat([p.rtab for p in plans]),
+            p0.rtab if p0.diffuse_specular else _cat([p.rtab for p in plans]),
             _cat([p.dscal for p in plans], 1) if p0.diffuse_specular
             else None)
         return out

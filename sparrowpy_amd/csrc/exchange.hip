@@ -641,6 +641,124 @@ __global__ __launch_bounds__(256) void group_spread_kernel(
   }
 }
 
+// Spread of the diffuse + specular plan.  Its BRDF rows are the unit
+// vector e_m (a specular group, key m < D) and the all-ones row (the
+// destination's last group, its diffuse sum), so
+//   cur[j, d, b, t] = gsum[diffuse] + gsum[group with key d]
+// (just the diffuse sum where no group has key d): the float64 sum that
+// group_spread_kernel forms with multiply-adds by ones and zeros, bit for
+// bit, without the D multiply-adds per group.  A workgroup takes one
+// (destination, band, 256 bins); each lane one bin and all D directions,
+// every direction's group-sum load issued before any is used.  MODE as in
+// group_spread_kernel.
+template <int DP, int MODE>
+__global__ __launch_bounds__(256) void ds_spread_kernel(
+    int D, int B, int T, int n_tblk, int64_t nblocks,
+    const int64_t *__restrict__ grp_ptr, const int32_t *__restrict__ grp_key,
+    const float *__restrict__ gsum, const float *__restrict__ prev,
+    float *__restrict__ cur, float *__restrict__ tot_hi,
+    float *__restrict__ tot_lo) {
+  constexpr int K = 8;
+  __shared__ int32_t gmap[DP];  // direction -> specular group offset, or -1
+  const int64_t tile = xcd_tile(blockIdx.x, nblocks);
+  const int64_t P = nblocks / (B * n_tblk);
+  const int tb = (int)(tile % n_tblk);
+  const int64_t j = (tile / n_tblk) % P;
+  const int b = (int)(tile / (n_tblk * P));
+  const int t = tb * 256 + (int)threadIdx.x;
+  const uint32_t row_bytes = (uint32_t)T * 4u;
+  const int64_t g0 = grp_ptr[j], g1 = grp_ptr[j + 1];
+  for (int d = (int)threadIdx.x; d < DP; d += 256) gmap[d] = -1;
+  __syncthreads();
+  for (int64_t g = g0 + (int64_t)threadIdx.x; g < g1 - 1; g += 256) {
+    const int32_t k = grp_key[g];
+    if (k >= 0 && k < D) gmap[k] = (int32_t)(g - g0);
+  }
+  __syncthreads();
+  float v[DP];
+#pragma unroll
+  for (int d = 0; d < DP; ++d) {
+    const int32_t gi = __builtin_amdgcn_readfirstlane(gmap[d]);
+    const __amdgpu_buffer_rsrc_t rs =
+        make_rsrc(gsum + ((g0 + (gi > 0 ? gi : 0)) * B + b) * (int64_t)T,
+                  gi >= 0 ? row_bytes : 0u);
+    v[d] = buffer_load(rs, (uint32_t)t * 4u);
+  }
+  const __amdgpu_buffer_rsrc_t rd =
+      make_rsrc(gsum + ((g1 > g0 ? g1 - 1 : g0) * B + b) * (int64_t)T,
+                g1 > g0 ? row_bytes : 0u);
+  const double gd = (double)buffer_load(rd, (uint32_t)t * 4u);
+
+  const int64_t dstride = (int64_t)B * T;
+  const int64_t base = j * D * dstride + (int64_t)b * T;
+  const uint32_t span = (uint32_t)(((int64_t)(D - 1) * dstride + T) * 4);
+  const __amdgpu_buffer_rsrc_t rc = make_rsrc(cur + base, span);
+  auto offset = [&](int d) {
+    return (d < D && t < T) ? (uint32_t)((int64_t)d * dstride + t) * 4u
+                            : kNoOffset;
+  };
+  if (MODE == 0) {
+#pragma unroll
+    for (int d = 0; d < DP; ++d)
+      buffer_store((float)((double)v[d] + gd), rc, offset(d));
+    return;
+  }
+  const __amdgpu_buffer_rsrc_t rh = make_rsrc(tot_hi + base, span);
+  const __amdgpu_buffer_rsrc_t rl = make_rsrc(tot_lo + base, span);
+  const __amdgpu_buffer_rsrc_t rp =
+      make_rsrc(MODE == 2 ? prev + base : cur + base, MODE == 2 ? span : 0u);
+#pragma unroll
+  for (int d0 = 0; d0 < DP; d0 += K) {
+    uint32_t off[K];
+    float h[K], l[K], e[K];
+#pragma unroll
+    for (int q = 0; q < K; ++q) {
+      if (d0 + q >= DP) break;
+      off[q] = offset(d0 + q);
+      h[q] = buffer_load(rh, off[q]);
+      l[q] = buffer_load(rl, off[q]);
+      if (MODE == 2) e[q] = buffer_load(rp, off[q]);
+    }
+#pragma unroll
+    for (int q = 0; q < K; ++q) {
+      if (d0 + q >= DP) break;
+      const double a = (double)v[d0 + q] + gd;
+      double sum = (double)h[q] + (double)l[q];
+      if (MODE == 2) sum += (double)e[q];
+      sum += a;
+      const float hs = (float)sum;
+      buffer_store((float)a, rc, off[q]);
+      buffer_store(hs, rh, off[q]);
+      buffer_store((float)(sum - (double)hs), rl, off[q]);
+    }
+  }
+}
+
+template <int DP>
+int launch_ds_spread(int64_t P, int64_t D, int64_t B, int64_t T,
+                     const int64_t *grp_ptr, const int32_t *grp_key,
+                     const float *gsum, const float *prev, float *cur,
+                     float *tot_hi, float *tot_lo, hipStream_t s) {
+  const int64_t nt = ceil_div(T, (int64_t)256);
+  const int64_t nb = P * B * nt;
+  SPR_REQUIRE(nb < (int64_t)0x7fffffff, "exchange grid too large");
+  SPR_REQUIRE(D * B * T < ((int64_t)1 << 29),
+              "histogram too large for one patch (D*B*T >= 2^29)");
+#define SPR_DS_MODE(M)                                                        \
+  hipLaunchKernelGGL((ds_spread_kernel<DP, M>), dim3((unsigned)nb),           \
+                     dim3(256), 0, s, (int)D, (int)B, (int)T, (int)nt, nb,    \
+                     grp_ptr, grp_key, gsum, prev, cur, tot_hi, tot_lo)
+  if (!tot_hi)
+    SPR_DS_MODE(0);
+  else if (!prev)
+    SPR_DS_MODE(1);
+  else
+    SPR_DS_MODE(2);
+#undef SPR_DS_MODE
+  SPR_LAUNCH_CHECK();
+  return SPR_OK;
+}
+
 template <int DP, int R>
 int launch_step(int64_t P, int64_t D, int64_t B, int64_t T,
                 const int64_t *row_ptr, const int32_t *src,
@@ -1504,6 +1622,41 @@ extern "C" int spr_exchange_spread_grouped_pair(
   return spread_grouped("spr_exchange_spread_grouped_pair", P, D, B, T,
                         grp_ptr, grp_key, rtab, gsum, prev, cur, tot_hi,
                         tot_lo, stream);
+}
+
+extern "C" int spr_exchange_spread_ds(int64_t P, int64_t D, int64_t B,
+                                      int64_t T, const int64_t *grp_ptr,
+                                      const int32_t *grp_key,
+                                      const float *gsum, const float *prev,
+                                      float *cur, float *tot_hi,
+                                      float *tot_lo, void *stream) {
+  const int64_t DP = spr_padded_directions(D);
+  SPR_REQUIRE(DP > 0, "spr_exchange_spread_ds: D=%lld unsupported",
+              (long long)D);
+  SPR_REQUIRE(P > 0 && B > 0 && T > 0 && T < 0x3fffffff,
+              "spr_exchange_spread_ds: bad sizes");
+  SPR_REQUIRE(grp_ptr && grp_key && gsum && cur,
+              "spr_exchange_spread_ds: null pointer");
+  SPR_REQUIRE(!tot_hi == !tot_lo,
+              "spr_exchange_spread_ds: tot_hi and tot_lo go together");
+  SPR_REQUIRE(!prev || (tot_hi && prev != cur),
+              "spr_exchange_spread_ds: prev needs the total and a separate "
+              "buffer");
+  hipStream_t s = as_stream(stream);
+#define SPR_DSS(DPV)                                                          \
+  return launch_ds_spread<DPV>(P, D, B, T, grp_ptr, grp_key, gsum, prev, cur, \
+                               tot_hi, tot_lo, s)
+  switch (DP) {
+    case 1: SPR_DSS(1);
+    case 2: SPR_DSS(2);
+    case 4: SPR_DSS(4);
+    case 8: SPR_DSS(8);
+    case 16: SPR_DSS(16);
+    case 32: SPR_DSS(32);
+    case 64: SPR_DSS(64);
+  }
+#undef SPR_DSS
+  return fail(SPR_ERR_UNSUPPORTED, "spr_exchange_spread_ds: bad D");
 }
 
 extern "C" int spr_exchange_total_add(int64_t n, const float *x,

@@ -656,7 +656,14 @@ class ExchangeState:
         every other order, for this order and the previous one); does not
         swap prev and cur."""
         P, D, B, T = self.shape
-        if self.pending:
+        if plan.diffuse_specular:
+            pair = self.pending
+            call("spr_exchange_spread_ds", P, D, B, T, ptr(plan.grp_ptr),
+                 ptr(plan.grp_key), ptr(self._gsum(plan)),
+                 ptr(self.prev) if pair else None, ptr(self.cur),
+                 ptr(self.hi) if pair else None,
+                 ptr(self.lo) if pair else None, stream_ptr())
+        elif self.pending:
             call("spr_exchange_spread_grouped_pair", P, D, B, T,
                  ptr(plan.grp_ptr), ptr(plan.grp_key), ptr(plan.rtab),
                  ptr(self._gsum(plan)), ptr(self.prev), ptr(self.cur),

@@ -446,6 +446,49 @@ def test_ds_plan_matches_oracle(ops, per_wall):
     assert err <= EXCHANGE_RTOL
 
 
+@pytest.mark.parametrize("per_wall", [True, False])
+def test_ds_spread_matches_grouped_spread(ops, per_wall):
+    """spr_exchange_spread_ds (cur = diffuse sum + the direction's specular
+    sum) gives the grouped spread's histogram and running total bit for
+    bit, without the total, adding cur, and adding prev + cur; T = 700
+    spans partial 256-bin blocks."""
+    import torch
+    from sparrowpy_amd._hip import call, ptr, stream_ptr
+    r = _oracle_directional(testing.shoebox_room_stub(3, 2.5, 2), .5,
+                            per_wall=per_wall, seed=4)
+    plan = _ds_plan(ops, r, 343., 1 / 2000)
+    P, D, B, T = plan.P, plan.D, plan.B, 700
+    g = torch.Generator(device="cuda").manual_seed(1)
+    gsum = torch.rand((plan.n_groups, B, T), generator=g, device="cuda")
+    prev = torch.rand((P, D, B, T), generator=g, device="cuda")
+    hi0 = torch.rand((P, D, B, T), generator=g, device="cuda")
+    lo0 = hi0 * 1e-8
+    for mode in range(3):
+        outs = []
+        for name in ("grouped", "ds"):
+            cur = torch.full((P, D, B, T), -1.0, device="cuda")
+            hi, lo = hi0.clone(), lo0.clone()
+            tot = (None, None) if mode == 0 else (ptr(hi), ptr(lo))
+            pv = ptr(prev) if mode == 2 else None
+            if name == "ds":
+                call("spr_exchange_spread_ds", P, D, B, T, ptr(plan.grp_ptr),
+                     ptr(plan.grp_key), ptr(gsum), pv, ptr(cur), *tot,
+                     stream_ptr())
+            elif mode == 2:
+                call("spr_exchange_spread_grouped_pair", P, D, B, T,
+                     ptr(plan.grp_ptr), ptr(plan.grp_key), ptr(plan.rtab),
+                     ptr(gsum), pv, ptr(cur), *tot, stream_ptr())
+            else:
+                call("spr_exchange_spread_grouped", P, D, B, T,
+                     ptr(plan.grp_ptr), ptr(plan.grp_key), ptr(plan.rtab),
+                     ptr(gsum), ptr(cur), *tot, stream_ptr())
+            outs.append((cur, hi, lo))
+        torch.cuda.synchronize()
+        for x, y in zip(*outs):
+            assert torch.equal(x, y), f"mode {mode}"
+        assert (outs[1][0] >= 0).all()
+
+
 def test_ds_plan_rejects_other_brdfs(ops):
     """A BRDF that is not constant-plus-one-entry per row (directional
     scattering) is reported as such; the grouped plan still applies."""
