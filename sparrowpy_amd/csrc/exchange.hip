@@ -300,10 +300,10 @@ __global__ __launch_bounds__(BS) void exchange_step_kernel(
 // is summed from the other, so no register holding an in-flight load is
 // ever copied and the compiler can wait for exactly the chunk it needs.
 // Each lane owns R bins.  A source bin before the histogram start
-// (t < lag) is an out-of-range buffer offset and reads as 0; so does
-// every bin of an entry past the row end or whose delay reaches beyond
-// the tile (descriptor of size 0).  Requires T < 2^29, so that
-// t - lag >= -T maps to a byte offset >= 2^31.
+// (t < lag) is a negative bin whose wrapped byte offset (bin_offset) lies
+// beyond the row and reads as 0; so does every bin of an entry past the
+// row end or whose delay reaches beyond the tile (descriptor of size 0).
+// Requires T < 2^29, so that t - lag >= -T maps to an offset >= 2^31.
 constexpr int kMetaBlock = 64;
 
 struct EntryMeta {
@@ -359,7 +359,7 @@ __global__ __launch_bounds__(BS) void group_gather_kernel(
           make_rsrc(src, n < t_end ? row_bytes : 0u);
 #pragma unroll
       for (int r = 0; r < R; ++r)
-        v[c][r] = buffer_load(rs, (uint32_t)(tl + BS * r - n) * 4u);
+        v[c][r] = buffer_load(rs, bin_offset(tl + BS * r - n));
     }
   };
 
@@ -445,7 +445,7 @@ __global__ __launch_bounds__(BS) void ds_gather_kernel(
           make_rsrc(src, in_row && n < t_end ? row_bytes : 0u);
 #pragma unroll
       for (int r = 0; r < R; ++r)
-        v[c][r] = buffer_load(rs, (uint32_t)(tl + BS * r - n) * 4u);
+        v[c][r] = buffer_load(rs, bin_offset(tl + BS * r - n));
     }
   };
   double S[R], Sd[R];

@@ -43,6 +43,20 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void *p,
                                            (int)bytes, kBufferWord3);
 }
 
+// Byte offset 4k of bin k of a row, kept whole.  For k < 0 the wrapped
+// 32-bit value lies beyond every buffer used here and the load reads 0 --
+// but only if the instruction receives it as computed: the compiler would
+// otherwise move constant parts of the offset into the instruction's
+// immediate field, and the hardware does not range-check the wrapped
+// register offset plus that immediate as the 32-bit sum it stands for
+// (loads of a row's first bins through such a pair return 0).  The empty
+// asm makes the value opaque, so it is never split.
+__device__ __forceinline__ uint32_t bin_offset(int k) {
+  uint32_t off = (uint32_t)k * 4u;
+  asm("" : "+v"(off));
+  return off;
+}
+
 __device__ __forceinline__ float buffer_load(__amdgpu_buffer_rsrc_t r,
                                              uint32_t off) {
   return __builtin_bit_cast(

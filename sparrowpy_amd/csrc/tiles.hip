@@ -255,7 +255,7 @@ __global__ __launch_bounds__(256 * (JT / JW)) void tile_gather_kernel(
             make_rsrc(src, n < t_end ? row_bytes : 0u);
 #pragma unroll
         for (int r = 0; r < R; ++r)
-          v[c][k][r] = buffer_load(rs, (uint32_t)(tl + BS * r - n) * 4u);
+          v[c][k][r] = buffer_load(rs, bin_offset(tl + BS * r - n));
       }
     }
   };

@@ -642,9 +642,9 @@ class ExchangeState:
         if plan.tiles is not None:
             tp = plan.tiles
             call("spr_exchange_gather_tiled", P, tp.JT, B, T, tp.M,
-                 ptr(tp.tptr), ptr(plan.grp_ptr), ptr(tp.mrow), ptr(tp.mlag),
-                 ptr(tp.mw), ptr(self.prev), ptr(self._gsum(plan)),
-                 stream_ptr())
+                 ptr(tp.tptr), ptr(plan.grp_ptr), ptr(tp.mrow),
+                 ptr(tp.mlag), ptr(tp.mw), ptr(self.prev),
+                 ptr(self._gsum(plan)), stream_ptr())
             return
         call("spr_exchange_gather_grouped", P, B, T, plan.Vd,
              ptr(plan.row_ptr), ptr(plan.grp_ptr), ptr(plan.rowid),

@@ -122,6 +122,47 @@ def test_tiled_gather_bitexact(ops, name, per_wall, jt):
 
 
 @pytest.mark.parametrize("jt", [4, 8])
+def test_tiled_gather_bitexact_wide_delays(ops, jt):
+    """At 48 kHz delays exceed 256 bins, so a row's first bins fall inside
+    the workgroups' bin ranges, behind lanes whose bins precede the row
+    (t < delay): the row-by-row sums match a CPU sum and the tiled ones
+    match them bit for bit."""
+    import torch
+    r = _room("3x2.5x2")
+    gp, _ = _grouped_and_generic(ops, r, 343., 1 / 48000)
+    tp = ops.tile_plan(gp, jt, min_merge=0)
+    P, D, B, T = gp.P, gp.D, gp.B, 1500
+    st = _random_state(ops, P, D, B, T, seed=11)
+    st.gather(gp)
+    ref = st.gsum[:gp.n_groups].clone()
+    # delays past 256 bins put a row's first bins at the ends of the
+    # workgroups' bin ranges: check the row-by-row sums against a CPU sum
+    cpu = _gsum_cpu(gp, _np(st.prev), T)
+    np.testing.assert_allclose(_np(ref), cpu, rtol=1e-6, atol=1e-30)
+    st.gsum.fill_(-1.0)
+    gp.tiles = tp
+    st.gather(gp)
+    torch.cuda.synchronize()
+    assert torch.equal(st.gsum[:gp.n_groups], ref)
+
+
+def _gsum_cpu(gp, prev, T):
+    """gsum[g, b, t] = sum_{e in g} scal[b, e] prev[rowid[e], b, t - lag[e]]
+    in float64."""
+    P, D, B = gp.P, gp.D, gp.B
+    rows = prev.reshape(P * D, B, T).astype(np.float64)
+    gs, rowid = _np(gp.grp_start), _np(gp.rowid)
+    lag, scal = _np(gp.lag) & LAG_MASK, _np(gp.scal)
+    out = np.zeros((gp.n_groups, B, T))
+    g_of_e = np.searchsorted(gs, np.arange(gp.Vd), side="right") - 1
+    for e in range(gp.Vd):
+        n = int(lag[e])
+        if n < T:
+            out[g_of_e[e], :, n:] += scal[:, e, None] * rows[rowid[e], :, :T - n]
+    return out
+
+
+@pytest.mark.parametrize("jt", [4, 8])
 def test_tiled_exchange_matches_oracle(ops, jt):
     """Whole exchange on tiles: equal to the untiled grouped plan and
     within the exchange tolerance of the oracle's FP64 loop."""
