@@ -220,6 +220,33 @@ def test_class_path_uses_tiles(ops):
     np.testing.assert_array_equal(hists[0], hists[1])
 
 
+@pytest.mark.parametrize("kind", ["tiled", "grouped", "ds"])
+def test_exchange_long_delays_matches_oracle(ops, kind):
+    """Whole exchange at 48 kHz (delays of up to ~600 bins, past the
+    kernels' 256-bin lane strides) on the tiled, row-by-row grouped and
+    diffuse + specular plans, against the oracle's FP64 loop."""
+    from test_gpu_parity import _ds_plan
+    r = _room("3x2.5x2", seed=6)
+    r.init_source_energy(np.array([1.3, .8, .9]))
+    T, c, dt, K = 1500, 343., 1 / 48000, 4
+    if kind == "ds":
+        plan = _ds_plan(ops, r, c, dt)
+    else:
+        plan, _ = _grouped_and_generic(ops, r, c, dt)
+        if kind == "tiled":
+            plan.tiles = ops.tile_plan(plan, 4, min_merge=0)
+    P, D, B = r.energy_init_source.shape
+    st = ops.ExchangeState(P, D, B, T)
+    st.init(r.energy_init_source, r.distance_patches_to_source, c, dt)
+    for _ in range(K):
+        st.step(plan)
+    st.check()
+    ref = orc.exchange(T, r.energy_init_source, r.distance_patches_to_source,
+                       r.pair_distances(), r.fft, r.p2o, c, dt, K,
+                       r.visible_patches, nthreads=8)
+    assert rel_err_within_db(_np(st.finalize()), ref) <= EXCHANGE_RTOL
+
+
 @pytest.mark.parametrize("K", [1, 2, 3, 6, 7])
 def test_totals_added_in_pairs(ops, K):
     """Grouped steps add orders to the running total two at a time (an odd
