@@ -11,8 +11,10 @@ per order covers the whole class and results are bit-for-bit those of the
 per-instance runs (same entries, same per-row order).  The K orders of a
 class are captured once into a HIP graph and replayed; per class the plan
 kind (grouped or diffuse + specular) is picked by timing one order of each
-candidate on the class's largest scene (``autotune``); a grouped class
-gathers on destination tiles of ``tile_rows`` rows (``ops.tile_plan``).
+candidate on the class's largest scene (``autotune``; only that scene's
+plans are built for it, the others once, for the kind chosen); a grouped
+class gathers on destination tiles of ``tile_rows`` rows
+(``ops.tile_plan``).
 Classes larger than the memory budget are split into chunks.
 """
 from __future__ import annotations
@@ -162,32 +164,42 @@ class BatchedExchange:
             chunks.append(cur)
         return chunks
 
-    def _pick_kind(self, key, xs):
-        """Plan kind of a class: the faster of the grouped and diffuse +
-        specular plans (when every scene of the class supports it), timed
-        on one order of the class's largest scene; generic otherwise."""
-        kinds = [k for k in ("grouped", "ds")
-                 if all(self._plan(x, k) is not None for x in xs)]
-        if not kinds:
-            return "generic"
-        if len(kinds) == 1 or not self.autotune:
-            return kinds[0]
-        best, best_t = None, None
+    def _kinds(self, xs):
+        """Plan kinds to try for a class, best first: the grouped and
+        diffuse + specular plans ordered by the time of one order on the
+        class's largest scene (only that scene's plans are built for it),
+        then the generic plan, which every scene supports."""
         x_big = max(xs, key=lambda x: self.items[x].radiosity.n_patches)
-        for k in kinds:
-            plans = {x_big: self._plan(x_big, k)}
-            run = self._build([(x_big, 0)], plans)
+        times = {}
+        for k in ("grouped", "ds"):
+            plan = self._plan(x_big, k)
+            if plan is None:
+                continue
+            if not self.autotune:
+                times[k] = len(times)
+                continue
+            run = self._build([(x_big, 0)], {x_big: plan})
             st = run["state"]
             st.step(run["plan"])             # warm-up
             torch.cuda.synchronize()
             t0 = time.perf_counter()
             st.step(run["plan"])
             torch.cuda.synchronize()
-            t = time.perf_counter() - t0
+            times[k] = time.perf_counter() - t0
             del run, st
-            if best_t is None or t < best_t:
-                best, best_t = k, t
-        return best
+        return sorted(times, key=times.get) + ["generic"]
+
+    def _class_plans(self, xs):
+        """(kind, {scene: plan}) for the first kind every scene supports."""
+        for kind in self._kinds(xs):
+            plans = {}
+            for x in xs:
+                plans[x] = self._plan(x, kind)
+                if plans[x] is None:
+                    break
+            else:
+                return kind, plans
+        raise RuntimeError("no exchange plan for a class")  # unreachable
 
     def _build(self, chunk, plans):
         xs = [x for x, _ in chunk]
@@ -246,9 +258,8 @@ class BatchedExchange:
         out = {}
         self.updates = 0
         for key, xs in self._classes().items():
-            kind = self._pick_kind(key, xs)
+            kind, plans = self._class_plans(xs)
             self.choice[key] = kind
-            plans = {x: self._plan(x, kind) for x in xs}
             for chunk in self._chunks(xs, plans):
                 run = self._build(chunk, plans)
                 self._exchange(run)
