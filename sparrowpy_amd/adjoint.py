@@ -273,3 +273,30 @@ def log_energy_loss(H, target, dynamic_range_db=60.0):
     tm = torch.where(mask, t, torch.ones_like(t))
     d = torch.log10(Hm) - torch.log10(tm)
     return (d * d * mask).sum()
+
+
+def recover_materials(adj, target, alpha0, scattering0, spec_value,
+                      steps=200, lr=0.02, bounds=(0.01, 0.99), log=None):
+    """Fit per-BRDF, per-band absorption and scattering ([NBRDF, B]) to
+    target receiver histograms: Adam on ``log_energy_loss`` through the
+    adjoint exchange, the parameters clipped to ``bounds`` after each step.
+    Returns (alpha, scattering, losses per step)."""
+    kw = dict(dtype=torch.float64, device=adj.dev)
+    a = torch.tensor(np.asarray(alpha0), requires_grad=True, **kw)
+    s = torch.tensor(np.asarray(scattering0), requires_grad=True, **kw)
+    opt = torch.optim.Adam([a, s], lr=lr)
+    losses = []
+    for step in range(steps):
+        opt.zero_grad()
+        A, C = scattering_tables(a, s, spec_value)
+        loss = log_energy_loss(receiver_histograms(adj, A, C), target)
+        loss.backward()
+        opt.step()
+        with torch.no_grad():
+            a.clamp_(*bounds)
+            s.clamp_(*bounds)
+        losses.append(float(loss))
+        if log is not None and (step % 20 == 0 or step == steps - 1):
+            log(f"step {step}: loss {losses[-1]:.4e}")
+    return a.detach(), s.detach(), losses
+
