@@ -200,6 +200,64 @@ def adjoint_timing(log):
     return out
 
 
+def bench_inverse(args, log):
+    """Inverse materials on the directional shoebox: target histograms from
+    the scene's seeded per-patch absorption and scattering, then Adam steps
+    (``--steps``, default 5; 200 for the full study) through the adjoint
+    from perturbed materials.  A step = forward with checkpoints + adjoint +
+    Adam update of 2 x 1,072 x 6 parameters."""
+    import torch
+    from sparrowpy_amd import DirectionalRadiosityFast, adjoint
+    from sparrowpy_amd.coordinates import Coordinates, FrequencyData
+    sc = _scene("shoebox")
+    r = DirectionalRadiosityFast.from_polygon(sc.walls, sc.patch_size)
+    for idx, br in sc.brdf_groups:
+        r.set_wall_brdf(idx, br, sc.directions, sc.directions)
+    r.set_air_attenuation(FrequencyData(sc.air_attenuation, sc.frequencies))
+    r.bake_geometry()
+    adj = adjoint.ExchangeAdjoint(
+        r, Coordinates(*sc.sources[0]), Coordinates(*sc.receivers.T),
+        sc.speed_of_sound, sc.time_resolution, sc.duration, sc.max_order,
+        checkpoint_every=5)
+    ta, ts = sc.materials
+    sv, _ = adjoint.specular_factors(sc.directions)
+    kw = dict(dtype=torch.float64, device="cuda")
+    A, C = adjoint.scattering_tables(torch.tensor(ta, **kw),
+                                     torch.tensor(ts, **kw), sv)
+    target = adj.forward(A, C).to(torch.float64)
+    adj.release()
+    rng = np.random.default_rng(11)
+    a0 = np.clip(ta + rng.normal(0, .05, ta.shape), .01, .99)
+    s0 = np.clip(ts + rng.normal(0, .1, ts.shape), .01, .99)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    a, s, losses = adjoint.recover_materials(adj, target, a0, s0, sv,
+                                             steps=args.steps, lr=0.01,
+                                             log=log)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    a, s = a.cpu().numpy(), s.cpu().numpy()
+    return {
+        "metric": "inverse-materials Adam steps/s (forward + adjoint)",
+        "value": args.steps / wall, "unit": "steps/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": 0,
+        "ms_per_step": wall / args.steps * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 state, f64 accumulate",
+        "data": "synthetic (seeded directional shoebox)",
+        "config": {"workload": f"{sc.name}: P={adj.P}, D={adj.D}, "
+                               f"B={adj.B}, T={adj.T}, order {adj.K}, "
+                               f"checkpoints every 5, {adj.R} receivers, "
+                               f"{a.size + s.size} parameters"},
+        "loss_first": losses[0], "loss_last": losses[-1],
+        "loss_every_20": losses[::20],
+        "abs_err_alpha": [float(np.abs(a0 - ta).mean()),
+                          float(np.abs(a - ta).mean())],
+        "abs_err_scattering": [float(np.abs(s0 - ts).mean()),
+                               float(np.abs(s - ts).mean())],
+    }
+
+
 def bench_batched(args, log):
     """Many rooms and sources: the per-instance loop of the reference API
     against the batched, graph-captured exchange on the same baked scenes.
@@ -295,7 +353,8 @@ def main(argv=None):
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default="canyon",
-                    choices=["canyon", "shoebox", "small", "batched"])
+                    choices=["canyon", "shoebox", "small", "batched",
+                             "inverse"])
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="target CPU time of the oracle baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -326,10 +385,11 @@ def main(argv=None):
     from sparrowpy_amd import DirectionalRadiosityFast, ops
     from sparrowpy_amd.coordinates import Coordinates, FrequencyData
 
-    if args.config == "batched":
+    if args.config in ("batched", "inverse"):
         def blog(msg):
             print(f"[bench] {msg}", file=sys.stderr, flush=True)
-        print(json.dumps(bench_batched(args, blog)), flush=True)
+        fn = bench_batched if args.config == "batched" else bench_inverse
+        print(json.dumps(fn(args, blog)), flush=True)
         return 0
     sc = _scene(args.config)
     torch.cuda.init()

@@ -50,6 +50,9 @@ class Scene:
     time_resolution: float
     duration: float
     max_order: int
+    # per-BRDF (absorption, scattering) [n_brdf, B] where the scene draws
+    # them (None otherwise)
+    materials: tuple = None
 
     @property
     def n_samples(self):
@@ -103,11 +106,13 @@ def directional_shoebox(seed=2, n_receivers=4, size=(10., 8., 3.),
     freqs = OCTAVE_BANDS[1:1 + n_bands] if n_bands <= 7 else \
         OCTAVE_BANDS[:n_bands]
     walls = _split_walls_into_patches(shoebox_room_stub(X, Y, Z), patch_size)
-    groups = []
+    groups, alphas, scats = [], [], []
     for i in range(len(walls)):
         alpha = rng.uniform(0.02, 0.4, n_bands)
         s = rng.uniform(0.1, 0.9, n_bands)
         groups.append(([i], _brdf(dirs, s, alpha, freqs)))
+        alphas.append(alpha)
+        scats.append(s)
     recv = np.stack([rng.uniform(1.0, X - 1.0, n_receivers),
                      rng.uniform(1.0, Y - 1.0, n_receivers),
                      rng.uniform(1.0, Z - 1.0, n_receivers)], axis=-1)
@@ -118,7 +123,8 @@ def directional_shoebox(seed=2, n_receivers=4, size=(10., 8., 3.),
         brdf_groups=groups, directions=dirs,
         air_attenuation=air_attenuation_octaves(freqs), sources=src,
         receivers=recv, speed_of_sound=343., time_resolution=1 / rate,
-        duration=duration, max_order=max_order)
+        duration=duration, max_order=max_order,
+        materials=(np.array(alphas), np.array(scats)))
 
 
 def _facade_rects(x_len, z_len, holes):
