@@ -317,10 +317,11 @@ __global__ __launch_bounds__(256 * (JT / JW)) void tile_gather_kernel(
 
 // Launch shapes, measured on the street canyon (T = 6,000): JT = 4 (5.2 M
 // merged entries) with R = 4 bins per lane and one merged entry per chunk
-// (118 VGPRs, 4 waves per SIMD) takes 166 ms; R = 2 with two entries per
-// chunk 211 ms; R = 8, 184 ms.  JT = 8 (3.3 M merged entries, 38 % of the
-// slots empty) in two groups of 4 waves takes 213 ms: its lower traffic does
-// not pay for the empty slots' issue slots.
+// (4 waves per SIMD) takes 172 ms (166 ms before offsets were kept whole,
+// see bin_offset); R = 2 with two entries per chunk 211 ms; R = 8, 184 ms.
+// JT = 8 (3.3 M merged entries, 38 % of the slots empty) in two groups of
+// 4 waves takes 213 ms: its lower traffic does not pay for the empty
+// slots' issue slots.
 template <int R, int C, int JT, int JW, int MB>
 int launch_tile_shape(int64_t P, int64_t B, int64_t T, int64_t M,
                       const int64_t *tptr, const int64_t *grp_ptr,
