@@ -285,13 +285,9 @@ def bench_batched(args, log):
     log(f"baked {len(scs)} rooms ({n_inst} instances) in {bake_s:.1f} s")
     c, dt, dur, K = (scs[0].speed_of_sound, scs[0].time_resolution,
                      scs[0].duration, scs[0].max_order)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    ref = run_per_instance(items, c, dt, dur, K)
-    torch.cuda.synchronize()
-    loop_s = time.perf_counter() - t0
-    log(f"per-instance loop {loop_s:.2f} s")
-    torch.cuda.empty_cache()
+    # the batched run first: its chunk buffers are tens of GiB each, and
+    # the per-instance loop leaves the device heap fragmented (each room's
+    # plan is placed in the previous room's freed histograms)
     be = BatchedExchange(items, c, dt, dur, K)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -299,6 +295,14 @@ def bench_batched(args, log):
     torch.cuda.synchronize()
     batched_s = time.perf_counter() - t0
     log(f"batched {batched_s:.2f} s, classes {be.choice}")
+    be.pool = None
+    torch.cuda.empty_cache()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ref = run_per_instance(items, c, dt, dur, K)
+    torch.cuda.synchronize()
+    loop_s = time.perf_counter() - t0
+    log(f"per-instance loop {loop_s:.2f} s")
     err = 0.0
     for key, v in ref.items():
         a, b = out[key].double(), v.double()

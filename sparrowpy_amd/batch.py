@@ -116,6 +116,9 @@ class BatchedExchange:
                 dev).total_memory // 2
         self.budget = int(memory_budget)
         self.autotune = autotune
+        # one set of state buffers for all chunks (allocating each chunk's
+        # anew cost more than the batched exchange saved)
+        self.pool = ops.StatePool(dev)
         self.choice = {}          # class key -> plan kind
         self._receivers = {}
         for x, it in enumerate(self.items):
@@ -178,7 +181,8 @@ class BatchedExchange:
             if not self.autotune:
                 times[k] = len(times)
                 continue
-            run = self._build([(x_big, 0)], {x_big: plan})
+            # its own buffers: the pool is sized for the class's chunks
+            run = self._build([(x_big, 0)], {x_big: plan}, pooled=False)
             st = run["state"]
             st.step(run["plan"])             # warm-up
             torch.cuda.synchronize()
@@ -201,7 +205,22 @@ class BatchedExchange:
                 return kind, plans
         raise RuntimeError("no exchange plan for a class")  # unreachable
 
-    def _build(self, chunk, plans):
+    def _pool_sizes(self, PDB, G, B):
+        """Pool buffers one chunk takes, in elements: four [P,D,B,T]
+        histograms and the [max(G, 1),B,T] group sums."""
+        return [PDB * self.T] * 4 + [max(G, 1) * B * self.T]
+
+    def _chunk_sizes(self, chunk, plans):
+        PDB = G = B = 0
+        for x, _ in chunk:
+            b = self.items[x].radiosity._bake
+            B = b["n_bins"]          # one value for the class
+            PDB += self.items[x].radiosity.n_patches * b["D"] * B
+            p = plans[x]
+            G += p.n_groups if isinstance(p, ops.GroupedPlan) else 0
+        return self._pool_sizes(PDB, G, B)
+
+    def _build(self, chunk, plans, pooled=True):
         xs = [x for x, _ in chunk]
         offsets = np.concatenate(
             [[0], np.cumsum([self.items[x].radiosity.n_patches
@@ -218,7 +237,12 @@ class BatchedExchange:
             d0.append(r._dev["distance_patches_to_source"])
         e0d, d0 = _cat(e0d), _cat(d0)
         P, D, B = e0d.shape
-        st = ops.ExchangeState(P, D, B, self.T)
+        pool = None
+        if pooled:
+            G = plan.n_groups if isinstance(plan, ops.GroupedPlan) else 0
+            self.pool.begin(self._pool_sizes(P * D * B, G, B))
+            pool = self.pool
+        st = ops.ExchangeState(P, D, B, self.T, pool=pool)
         st.init(e0d, d0, self.c, self.dt)
         if isinstance(plan, ops.GroupedPlan):
             st._gsum(plan)
@@ -257,10 +281,19 @@ class BatchedExchange:
         """{(scene, source): receiver histograms float32 [R, B, T]}."""
         out = {}
         self.updates = 0
+        work = []
         for key, xs in self._classes().items():
             kind, plans = self._class_plans(xs)
             self.choice[key] = kind
-            for chunk in self._chunks(xs, plans):
+            work.append((plans, self._chunks(xs, plans)))
+        # the pool is sized once, for the largest chunk of every class: a
+        # freed buffer of tens of GiB is soon pinned by small allocations
+        # placed in its tail, so growing it later would need the memory twice
+        sizes = [self._chunk_sizes(c, plans) for plans, chunks in work
+                 for c in chunks]
+        self.pool.begin([max(z) for z in zip(*sizes)])
+        for plans, chunks in work:
+            for chunk in chunks:
                 run = self._build(chunk, plans)
                 self._exchange(run)
                 st = run["state"]
@@ -272,6 +305,7 @@ class BatchedExchange:
                     slab = total[int(off[n]):int(off[n + 1])]
                     out[(x, s)] = ops.collect(slab, pts, centers, idx,
                                               factor, air, self.c, self.dt)
+                    del slab      # a view of the pool
                 plan = run["plan"]
                 lag = plan.lag.to(torch.int64) & (
                     ops.GroupedPlan.LAST_IN_GROUP - 1)

@@ -575,20 +575,69 @@ def diffuse_specular_plan(vis, patches_center, patches_area, form_factors,
     return plan
 
 
+class StatePool:
+    """Device buffers that successive ExchangeStates (the chunks of a
+    batched run) take their histograms from, instead of allocating their
+    own.  ``begin(sizes)`` makes buffer ``k`` hold at least ``sizes[k]``
+    float32 and rewinds; the ``k``-th ``take`` after it returns a view of
+    buffer ``k``.  A later ``begin`` reuses the memory, so only one state
+    built from the pool is live at a time.  Separate buffers (one per
+    histogram) rather than one block: a device heap that earlier work left
+    fragmented can still place pieces of this size."""
+
+    def __init__(self, dev=None):
+        self.dev = dev or device()
+        self.bufs = []
+        self.next = 0
+
+    def begin(self, sizes):
+        """Rewind; when a buffer must grow, every buffer is freed (and the
+        allocator's cache emptied) before any is allocated again, at the
+        larger of its old and new size, so old and new never coexist."""
+        sizes = [max(int(n), 1) for n in sizes]
+        old = [b.numel() for b in self.bufs]
+        if len(old) < len(sizes) or any(n > m for n, m in zip(sizes, old)):
+            keep = [max(n, old[k] if k < len(old) else 0)
+                    for k, n in enumerate(sizes)] + old[len(sizes):]
+            self.bufs = []
+            torch.cuda.empty_cache()
+            self.bufs = [torch.empty(n, dtype=torch.float32, device=self.dev)
+                         for n in keep]
+        self.next = 0
+
+    def take(self, n):
+        """A float32 view of ``n`` elements of the next buffer."""
+        k = self.next
+        if k >= len(self.bufs) or n > self.bufs[k].numel():
+            raise RuntimeError(
+                "StatePool: take %d of %d elements from buffer %d of %d"
+                % (n, self.bufs[k].numel() if k < len(self.bufs) else 0, k,
+                   len(self.bufs)))
+        self.next += 1
+        return self.bufs[k][:int(n)]
+
+
 class ExchangeState:
     """Order histograms and the compensated running total of one source.
 
     ``prev``/``cur`` are float32 [P,D,B,T]; the total is ``hi + lo``.
+    With ``pool`` (a StatePool, rewound by its owner) the buffers, and the
+    grouped plan's workspace, are slices of the pool's buffer.
     """
 
-    def __init__(self, P, D, B, T, dev=None):
+    def __init__(self, P, D, B, T, dev=None, pool=None):
         dev = dev or device()
         shape = (P, D, B, T)
         self.shape = shape
-        self.prev = torch.empty(shape, dtype=torch.float32, device=dev)
-        self.cur = torch.empty(shape, dtype=torch.float32, device=dev)
-        self.hi = torch.empty(shape, dtype=torch.float32, device=dev)
-        self.lo = torch.empty(shape, dtype=torch.float32, device=dev)
+        self.pool = pool
+        n = P * D * B * T
+
+        def buf(name):
+            if pool is not None:
+                return pool.take(n).view(shape)
+            return torch.empty(shape, dtype=torch.float32, device=dev)
+        self.prev, self.cur = buf("prev"), buf("cur")
+        self.hi, self.lo = buf("hi"), buf("lo")
         self.err = torch.zeros(1, dtype=torch.int32, device=dev)
         self.gsum = None  # per-group sums of the grouped plan (workspace)
         # grouped steps add orders to the running total in pairs; True while
@@ -626,8 +675,13 @@ class ExchangeState:
         P, D, B, T = self.shape
         G = plan.n_groups
         if self.gsum is None or self.gsum.shape[0] < max(G, 1):
-            self.gsum = torch.empty((max(G, 1), B, T), dtype=torch.float32,
-                                    device=self.prev.device)
+            if self.pool is not None:
+                self.gsum = self.pool.take(max(G, 1) * B * T).view(
+                    max(G, 1), B, T)
+            else:
+                self.gsum = torch.empty((max(G, 1), B, T),
+                                        dtype=torch.float32,
+                                        device=self.prev.device)
         return self.gsum
 
     def gather(self, plan):
