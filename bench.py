@@ -382,7 +382,7 @@ def main(argv=None):
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not "
               "supported (the exchange shards by band with no collectives; "
-              "multi-GPU runs are not part of this build)", file=sys.stderr)
+              "only the one-GPU measurement is implemented)", file=sys.stderr)
         return 2
 
     import torch

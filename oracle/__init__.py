@@ -1,7 +1,8 @@
 """FP64 CPU oracle of sparrowpy's DirectionalRadiosityFast hot path.
 
 TEST INFRASTRUCTURE ONLY.  Imported by ``tests/``, the package smoke check and
-the benchmark's CPU-baseline leg as the checker.  The product package
+the benchmark's CPU-baseline leg, as the values the GPU results are compared
+against (and, in the benchmark, as the CPU timing).  The product package
 (``sparrowpy_amd``) never imports it.
 
 ``oracle.c_*`` wrap the C restatement in ``sparrow_oracle.c`` (one function

@@ -2,7 +2,8 @@
  * FP64 CPU oracle of sparrowpy's DirectionalRadiosityFast hot path.
  *
  * TEST INFRASTRUCTURE ONLY: used by tests/, the package smoke check and the
- * benchmark's CPU-baseline leg as the checker; never by the product.
+ * benchmark's CPU-baseline leg, as the values the GPU results are compared
+ * against; never by the product.
  *
  * This is a statement-by-statement restatement of the reference numerics
  * (sparrowpy 1.0.1).  Operation order follows the Python source: a.b for

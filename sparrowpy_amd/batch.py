@@ -181,7 +181,7 @@ class BatchedExchange:
             if not self.autotune:
                 times[k] = len(times)
                 continue
-            # its own buffers: the pool is sized for the class's chunks
+            # its own buffers: the pool is sized for the chunks
             run = self._build([(x_big, 0)], {x_big: plan}, pooled=False)
             st = run["state"]
             st.step(run["plan"])             # warm-up

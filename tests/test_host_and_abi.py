@@ -133,3 +133,26 @@ def test_ops_fail_loudly_without_gpu():
         ops.check_patch2patch_visibility(np.zeros((2, 3)),
                                          np.zeros((2, 3)),
                                          np.zeros((2, 4, 3)))
+
+
+def test_state_pool_reuses_and_grows():
+    """StatePool (the batched run's chunk buffers) on CPU tensors: takes
+    are views of the k-th buffer, a rewind reuses the same memory, growth
+    replaces every buffer at the larger size, and over-asking raises."""
+    import torch
+    from sparrowpy_amd import ops
+    pool = ops.StatePool(torch.device("cpu"))
+    pool.begin([10, 10, 4])
+    a, b, c = pool.take(10), pool.take(7), pool.take(4)
+    assert a.data_ptr() != b.data_ptr() != c.data_ptr()
+    assert [t.numel() for t in (a, b, c)] == [10, 7, 4]
+    with pytest.raises(RuntimeError):
+        pool.take(1)                 # no fourth buffer
+    first = [t.data_ptr() for t in pool.bufs]
+    pool.begin([8, 10, 2])           # fits: same memory, rewound
+    assert [t.data_ptr() for t in pool.bufs] == first
+    assert pool.take(8).data_ptr() == first[0]
+    with pytest.raises(RuntimeError):
+        pool.take(11)                # beyond buffer 1
+    pool.begin([12, 3])              # grows: old sizes kept where larger
+    assert [t.numel() for t in pool.bufs] == [12, 10, 4]
