@@ -74,8 +74,15 @@ def test_graph_replays_and_chunks(items):
     from sparrowpy_amd.batch import BatchedExchange
     a = _np(BatchedExchange(items, C, DT, DUR, K, autotune=False).run())
     b = _np(BatchedExchange(items, C, DT, DUR, K, autotune=False).run())
-    c = _np(BatchedExchange(items, C, DT, DUR, K, autotune=False,
-                            memory_budget=1).run())
+    # one instance per chunk: chunks of different sizes share the state
+    # buffers, sized once; a second run of the same object reuses them
+    be = BatchedExchange(items, C, DT, DUR, K, autotune=False,
+                         memory_budget=1)
+    c = _np(be.run())
+    ptrs = [t.data_ptr() for t in be.pool.bufs]
+    d = _np(be.run())
+    assert [t.data_ptr() for t in be.pool.bufs] == ptrs
     for key in a:
         np.testing.assert_array_equal(a[key], b[key])
         np.testing.assert_array_equal(a[key], c[key])
+        np.testing.assert_array_equal(a[key], d[key])
