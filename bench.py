@@ -505,12 +505,15 @@ def main(argv=None):
         spread_bytes = (4 * G * B * T + 4 * n_state +
                         20 * n_state * n_pair / args.steps)
         gather_name = ("tile_gather_kernel" if tiles is not None else
+                       "ds_gather_kernel" if kind == "ds" else
                        "group_gather_kernel")
+        spread_name = ("ds_spread_kernel" if kind == "ds" else
+                       "group_spread_kernel")
         kernels = {
             gather_name: {
                 "ms": gather_s * 1e3, "algorithmic_bytes": gather_bytes,
                 "GB/s": gather_bytes / gather_s / 1e9},
-            "group_spread_kernel": {
+            spread_name: {
                 "ms": spread_s * 1e3, "algorithmic_bytes": spread_bytes,
                 "GB/s": spread_bytes / spread_s / 1e9},
         }
