@@ -177,14 +177,31 @@ class DevicePointer(ctypes.c_void_p):
         self.tensor = tensor
 
 
+_anchors = {}
+
+
+def _anchor(device):
+    """A small device buffer whose address stands for zero-length arrays."""
+    a = _anchors.get(device)
+    if a is None:
+        a = _anchors[device] = torch.zeros(64, dtype=torch.uint8,
+                                           device=device)
+    return a
+
+
 def ptr(t):
-    """Device pointer of a tensor (None -> NULL)."""
+    """Device pointer of a tensor (None -> NULL).  A zero-length tensor,
+    whose own address may be NULL, gets a valid address that nothing reads
+    (its length, passed beside it, is 0), so entry points can keep
+    rejecting NULL for arrays that are required."""
     if t is None:
         return None
     if not t.is_cuda:
         raise ValueError("expected a device tensor")
     if not t.is_contiguous():
         raise ValueError("expected a contiguous tensor")
+    if t.numel() == 0:
+        return DevicePointer(_anchor(t.device))
     return DevicePointer(t)
 
 

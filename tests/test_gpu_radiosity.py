@@ -322,3 +322,40 @@ def test_pipeline_parity_directional_canyon(sp):
     e_g = g.collect_energy_receiver_patchwise(Coordinates(*recv.T)).time
     e_o = np.stack([o.collect_energy_receiver_patchwise(p)[0] for p in recv])
     assert rel_err_within_db(e_g, e_o) <= 1e-5
+
+
+def test_pipeline_no_visible_pairs(sp):
+    """One flat floor: no patch sees another (no visible pairs), so the
+    exchange adds nothing to order 0; histogram and receiver collection
+    still match the oracle, through every plan kind."""
+    floor = [w for w in testing.shoebox_room_stub(4, 3, 2)
+             if np.allclose(w.normal, [0, 0, 1])]
+    assert len(floor) == 1
+    freqs = np.array([1000.])
+
+    def conf(g, o):
+        br, c = _diffuse_brdf(freqs, .2)
+        g.set_wall_brdf([0], br, c, c)
+        o.set_wall_brdf([0], br.freq, c.cartesian, c.cartesian, freqs)
+        g.set_air_attenuation(FrequencyData(np.zeros(1), freqs))
+        o.set_air_attenuation(np.zeros(1), freqs)
+
+    g, o = _both(sp, floor, 1., conf)
+    g.bake_geometry()
+    o.bake_geometry()
+    assert not np.asarray(g._visibility_matrix).any()
+    src = np.array([2., 1.5, 1.])
+    g.init_source_energy(Coordinates(*src))
+    o.init_source_energy(src)
+    o.calculate_energy_exchange(343., 1 / 1000, .1, 3, nthreads=8)
+    rec = np.array([1., 1., 1.2])
+    e_o = o.collect_energy_receiver_mono(rec)
+    for kind in ("grouped", "ds", "generic"):
+        g._plan = None
+        plan = g._exchange_plan(343., 1 / 1000, kind)
+        assert plan.Vd == 0
+        g._plan = (g._plan[0][:2] + (None,) + g._plan[0][3:], plan)
+        g.calculate_energy_exchange(343., 1 / 1000, .1, 3, recalculate=True)
+        assert rel_err_within_db(g._energy_exchange_etc, o.etc) <= 1e-5, kind
+        e_g = g.collect_energy_receiver_mono(Coordinates(*rec)).time
+        assert rel_err_within_db(e_g, e_o) <= 1e-5, kind
