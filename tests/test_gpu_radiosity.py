@@ -359,3 +359,35 @@ def test_pipeline_no_visible_pairs(sp):
         assert rel_err_within_db(g._energy_exchange_etc, o.etc) <= 1e-5, kind
         e_g = g.collect_energy_receiver_mono(Coordinates(*rec)).time
         assert rel_err_within_db(e_g, e_o) <= 1e-5, kind
+
+
+def test_exchange_runs_once_unless_recalculated(sp):
+    """RadiosityFast.py:547: a second calculate_energy_exchange keeps the
+    first histogram (its arguments unused) unless recalculate=True."""
+    r = _setup_diffuse(sp, testing.shoebox_room_stub(2, 2, 2), 1., [1000.],
+                       .1)
+    r.bake_geometry()
+    r.init_source_energy(Coordinates(.5, .6, .7))
+    r.calculate_energy_exchange(343., 1 / 1000, .05, 2)
+    first = r._energy_exchange_etc.copy()
+    r.calculate_energy_exchange(343., 1 / 1000, .05, 6)
+    np.testing.assert_array_equal(r._energy_exchange_etc, first)
+    r.calculate_energy_exchange(343., 1 / 1000, .05, 6, recalculate=True)
+    again = r._energy_exchange_etc
+    assert again.shape == first.shape
+    assert again.sum() > first.sum()        # orders 3..6 added
+
+
+def test_source_outside_closed_room(sp):
+    """A source outside a closed room (test_visibility.py:209-224 for the
+    visibility) reaches no patch: every histogram stays zero through the
+    exchange and the collection."""
+    r = _setup_diffuse(sp, testing.shoebox_room_stub(2, 2, 2), 1., [1000.],
+                       .1)
+    r.bake_geometry()
+    r.init_source_energy(Coordinates(5., 6., 7.))
+    assert not r._source_visibility.any()
+    r.calculate_energy_exchange(343., 1 / 1000, .05, 3)
+    assert not r._energy_exchange_etc.any()
+    e = r.collect_energy_receiver_mono(Coordinates(1., 1., 1.)).time
+    assert e.shape == (1, 1, 50) and not np.asarray(e).any()
