@@ -86,3 +86,38 @@ def test_graph_replays_and_chunks(items):
         np.testing.assert_array_equal(a[key], b[key])
         np.testing.assert_array_equal(a[key], c[key])
         np.testing.assert_array_equal(a[key], d[key])
+
+
+def test_room_without_visible_pairs_in_a_class(items):
+    """A flat floor (no patch sees another: an empty plan) batched in one
+    class with a closed room, in one chunk and in one chunk per instance,
+    against the per-instance loop."""
+    from sparrowpy_amd import DirectionalRadiosityFast
+    from sparrowpy_amd.batch import (BatchedExchange, BatchItem,
+                                     run_per_instance)
+    d16 = samplings.hemisphere(samplings.sph_gaussian(sh_order=3))
+    freqs = np.array([500.])
+
+    def room(walls, src, rec):
+        r = DirectionalRadiosityFast.from_polygon(walls, 1.0)
+        br = B.create_from_scattering(
+            d16, d16, FrequencyData(np.array([[.5]]), freqs),
+            FrequencyData(np.array([[.1]]), freqs))
+        r.set_wall_brdf(np.arange(len(walls)), br, d16, d16)
+        r.set_air_attenuation(FrequencyData(np.array([1e-3]), freqs))
+        r.bake_geometry()
+        return BatchItem(r, np.array(src), np.array(rec))
+
+    floor = [w for w in testing.shoebox_room_stub(4, 3, 2)
+             if np.allclose(w.normal, [0, 0, 1])]
+    its = [room(floor, [[2., 1.5, 1.]], [[1., 1., 1.2]]),
+           room(testing.shoebox_room_stub(3, 3, 2.5), [[1., 1., 1.]],
+                [[2., 2., 1.5]])]
+    assert not np.asarray(its[0].radiosity._visibility_matrix).any()
+    ref = _np(run_per_instance(its, C, DT, DUR, K))
+    for budget in (None, 1):
+        out = _np(BatchedExchange(its, C, DT, DUR, K,
+                                  memory_budget=budget).run())
+        assert set(out) == set(ref)
+        for key in ref:
+            assert rel_err_within_db(out[key], ref[key]) <= 1e-6, key
