@@ -7,9 +7,8 @@ import re
 import numpy as np
 import pytest
 
-from sparrowpy_amd import brdf as B
 from sparrowpy_amd import geometry, samplings, scenes, testing
-from sparrowpy_amd.coordinates import Coordinates, FrequencyData
+from sparrowpy_amd.coordinates import Coordinates
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 HEADER = os.path.join(ROOT, "include", "sparrow_hip.h")

@@ -3,9 +3,7 @@ differences of the FP64 oracle forward path (h = 1e-4), on a small room
 with one BRDF per patch: what pins the reference the GPU adjoint is
 checked against."""
 import numpy as np
-import pytest
 
-import oracle as orc
 from oracle import adjoint as oadj
 from conftest import make_oracle
 from sparrowpy_amd import brdf as B
