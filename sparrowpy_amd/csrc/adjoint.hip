@@ -354,8 +354,9 @@ extern "C" int spr_exchange_adjoint_ds(
               "spr_exchange_adjoint_ds: muout aliases an input");
   // 8 bins per lane, 2 entries per chunk: each entry's two wave
   // reductions (its table-gradient dot products) then cover 512 bins.
-  // Directional shoebox, order 30: backward 1.71 s with this shape, 1.78 s
-  // with R = 4, C = 2 and 2.68 s with R = 2, C = 4.
+  // Directional shoebox, order 30: backward 1.49 s with this shape; 1.66 s
+  // with R = 4, C = 4, 1.76 s with R = 8, C = 1, 1.78 s with R = 16, C = 1
+  // or R = 4, C = 2, 1.95 s with R = 16, C = 2, 2.68 s with R = 2, C = 4.
   constexpr int R = 8, C = 2;
   const int64_t nt = ceil_div(T, (int64_t)256 * R);
   const int64_t nb = P * B * nt;
