@@ -359,6 +359,14 @@ def test_pipeline_no_visible_pairs(sp):
         assert rel_err_within_db(g._energy_exchange_etc, o.etc) <= 1e-5, kind
         e_g = g.collect_energy_receiver_mono(Coordinates(*rec)).time
         assert rel_err_within_db(e_g, e_o) <= 1e-5, kind
+    # the reference-shaped entry point with an empty pair list
+    from sparrowpy_amd import ops
+    assert len(o.visible_patches) == 0
+    etc = ops.energy_exchange(100, o.energy_init_source,
+                              o.distance_patches_to_source,
+                              o.pair_distances(), o.fft, o.p2o, 343.,
+                              1 / 1000, 3, o.visible_patches)
+    assert rel_err_within_db(etc.cpu().numpy(), o.etc) <= 1e-5
 
 
 def test_exchange_runs_once_unless_recalculated(sp):
