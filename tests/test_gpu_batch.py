@@ -90,8 +90,8 @@ def test_graph_replays_and_chunks(items):
 
 def test_room_without_visible_pairs_in_a_class(items):
     """A flat floor (no patch sees another: an empty plan) batched in one
-    class with a closed room, in one chunk and in one chunk per instance,
-    against the per-instance loop."""
+    class with a closed room (and a room without sources), in one chunk
+    and in one chunk per instance, against the per-instance loop."""
     from sparrowpy_amd import DirectionalRadiosityFast
     from sparrowpy_amd.batch import (BatchedExchange, BatchItem,
                                      run_per_instance)
@@ -112,9 +112,13 @@ def test_room_without_visible_pairs_in_a_class(items):
              if np.allclose(w.normal, [0, 0, 1])]
     its = [room(floor, [[2., 1.5, 1.]], [[1., 1., 1.2]]),
            room(testing.shoebox_room_stub(3, 3, 2.5), [[1., 1., 1.]],
-                [[2., 2., 1.5]])]
+                [[2., 2., 1.5]]),
+           # a room without sources contributes no instance
+           room(testing.shoebox_room_stub(2, 2, 2), np.zeros((0, 3)),
+                [[1., 1., 1.]])]
     assert not np.asarray(its[0].radiosity._visibility_matrix).any()
     ref = _np(run_per_instance(its, C, DT, DUR, K))
+    assert set(ref) == {(0, 0), (1, 0)}
     for budget in (None, 1):
         out = _np(BatchedExchange(its, C, DT, DUR, K,
                                   memory_budget=budget).run())
