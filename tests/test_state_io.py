@@ -143,3 +143,32 @@ def test_io_within_simulation(gpu, tmp_path):
     rec = Coordinates(.3, .4, .6)
     assert rel_err_within_db(rr.collect_energy_receiver_mono(rec).time,
                              r.collect_energy_receiver_mono(rec).time) <= 1e-6
+
+
+@pytest.mark.gpu
+def test_io_without_visible_pairs(gpu, tmp_path):
+    """A single floor: no patch sees another.  Every stage survives the
+    round trip, and a simulation continued from the saved state (which
+    runs on the saved dense F~, here without a visible pair) gives the
+    uninterrupted run's histogram."""
+    floor = [w for w in testing.shoebox_room_stub(2, 2, 2)
+             if np.allclose(w.normal, [0, 0, 1])]
+    r = DirectionalRadiosityFast.from_polygon(floor, 1)
+    c = Coordinates(0, 0, 1, weights=1)
+    freqs = np.array([1000.])
+    br = B.create_from_scattering(c, c, FrequencyData(1, freqs),
+                                  FrequencyData(.1, freqs))
+    r.set_wall_brdf([0], br, c, c)
+    r.set_air_attenuation(FrequencyData(np.zeros_like(freqs), freqs))
+    r.bake_geometry()
+    assert not np.asarray(r._visibility_matrix).any()
+    assert _roundtrip(r, tmp_path / "1.npz") == r
+    r.init_source_energy(Coordinates(.5, .5, .5))
+    rr = _roundtrip(r, tmp_path / "2.npz")
+    assert rr == r
+    r.calculate_energy_exchange(343, 1 / 1000, .05, 3)
+    rr.calculate_energy_exchange(343, 1 / 1000, .05, 3)
+    np.testing.assert_array_equal(rr._energy_exchange_etc,
+                                  r._energy_exchange_etc)
+    assert r._energy_exchange_etc.any()
+    assert _roundtrip(r, tmp_path / "3.npz") == r
