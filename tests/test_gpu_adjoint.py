@@ -167,3 +167,45 @@ def test_adam_recovers_materials(torch):
     assert losses[-1] < 1e-2 * losses[0]
     assert err1 < 0.9 * err0
 
+
+
+def test_gradient_without_visible_pairs(torch):
+    """A floor split into patches (no patch sees another, so no transposed
+    entries): the gradient flows only through injection and collection,
+    and still matches the FP64 reference."""
+    from sparrowpy_amd import adjoint, samplings, scenes, testing
+    floor = [w for w in testing.shoebox_room_stub(2, 1.5, 1)
+             if np.allclose(w.normal, [0, 0, 1])]
+    walls = scenes._split_walls_into_patches(floor, .5)
+    dirs = samplings.hemisphere(samplings.sph_gaussian(sh_order=3))
+    n = len(walls)
+    rng = np.random.default_rng(1)
+    ta, ts = rng.uniform(.05, .4, (n, 2)), rng.uniform(.1, .9, (n, 2))
+    alpha = np.clip(ta + rng.normal(0, .05, (n, 2)), .02, .6)
+    scat = np.clip(ts + rng.normal(0, .1, (n, 2)), .05, .95)
+    src = np.array([.6, .5, .4])
+    rcv = np.array([[1.4, 1.0, .6], [.9, .4, .7]])
+    T, c, dt, K = 120, 343., 1 / 2000, 3
+    target = run(oracle_scene(walls, dirs, ta, ts), src, rcv, T, c, dt, K)
+    ro = oracle_scene(walls, dirs, alpha, scat)
+    assert len(ro.visible_patches) == 0
+    L, gH = loss_and_grad(run(ro, src, rcv, T, c, dt, K), target)
+    sv, m = spec_info(dirs)
+    ga_ref, gs_ref = oadj.scattering_gradient(
+        oadj.gradient(ro, rcv, gH, K), alpha, scat, sv, m)
+    _, adj = _adjoint(torch, walls, dirs, alpha, scat, src, rcv, T, c, dt, K,
+                      2)
+    a = torch.tensor(alpha, dtype=torch.float64, device="cuda",
+                     requires_grad=True)
+    s = torch.tensor(scat, dtype=torch.float64, device="cuda",
+                     requires_grad=True)
+    A, C = adjoint.scattering_tables(a, s, sv)
+    loss = adjoint.log_energy_loss(adjoint.receiver_histograms(adj, A, C),
+                                   target)
+    loss.backward()
+    ga, gs = a.grad.cpu().numpy(), s.grad.cpu().numpy()
+    gmax = max(np.abs(ga_ref).max(), np.abs(gs_ref).max())
+    assert gmax > 0
+    assert abs(float(loss) - L) <= 1e-5 * L
+    assert max(np.abs(ga - ga_ref).max(), np.abs(gs - gs_ref).max()) <= \
+        1e-4 * gmax
