@@ -155,3 +155,17 @@ def test_state_pool_reuses_and_grows():
         pool.take(11)                # beyond buffer 1
     pool.begin([12, 3])              # grows: old sizes kept where larger
     assert [t.numel() for t in pool.bufs] == [12, 10, 4]
+
+
+def test_bench_refuses_several_gpus():
+    """bench.py measures one GPU: --gpus 2 exits with status 2 and says
+    why, before importing torch or touching a device."""
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([sys.executable, os.path.join(root, "bench.py"),
+                        "--gpus", "2"], capture_output=True, text=True,
+                       timeout=60)
+    assert r.returncode == 2
+    assert "one GPU" in r.stderr
+    assert r.stdout == ""
