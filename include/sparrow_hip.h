@@ -55,7 +55,8 @@ int spr_p2p_visibility(int64_t P, const double *centers, int64_t NS,
                        void *stream);
 
 /* Workspace bytes of spr_p2p_visibility / spr_point_visibility for NS
- * blocking surfaces (one precomputed record per surface). */
+ * blocking surfaces (one precomputed record per surface).
+ * Sizing helper; no counterpart in the reference. */
 int64_t spr_visibility_workspace(int64_t NS);
 
 /* geometry._check_point2patch_visibility (sparrowpy/geometry.py:799-839),
@@ -161,13 +162,16 @@ int spr_source_energy(const double *source, int64_t P, const double *centers,
  * ------------------------------------------------------------------- */
 
 /* Directions padded to the kernel's register tile (1,2,4,...,64); 0 when
- * D is unsupported (D > 64). */
+ * D is unsupported (D > 64).
+ * Sizing helper; no counterpart in the reference. */
 int64_t spr_padded_directions(int64_t D);
 
 /* Plan structure from the visibility matrix: row_ptr and src.
  * Pass 1 (src == NULL) writes row_ptr (row_ptr[P] = Vd); pass 2 fills src.
  * The rows list, for destination j, every i != j with vis[min(i,j),
- * max(i,j)] = 1 in ascending order. */
+ * max(i,j)] = 1 in ascending order.
+ * Replaces the visible-pair walk of _energy_exchange
+ * (RadiosityFast.py:1124-1143). */
 int spr_plan_rows(int64_t P, const uint8_t *vis, int64_t *row_ptr,
                   int32_t *src, void *stream);
 
@@ -235,7 +239,9 @@ int spr_plan_grouped_sort(int64_t P, int64_t max_row, const int64_t *row_ptr,
  * lag[Vd] (int32: the delay, with bit 30 set on the last entry of each
  * group), scal[B,Vd] (float64, band-major, F~_ij * exp(-air_b |u_ij|)) and
  * rtab[NK,B,DP] (float64, the BRDF rows with NK = NBRDF*S; brdf may be NULL
- * with D = NK = 1). */
+ * with D = NK = 1).
+ * Replaces, with spr_plan_grouped_sort, the per-pair terms of one order of
+ * _energy_exchange, RadiosityFast.py:1121-1144. */
 int spr_plan_grouped_fill(int64_t P, int64_t D, int64_t B, double c,
                           double dt, int64_t Vd, const int64_t *row_ptr,
                           const int32_t *src, const int32_t *key,
@@ -247,7 +253,9 @@ int spr_plan_grouped_fill(int64_t P, int64_t D, int64_t B, double c,
                           double *scal, double *rtab, void *stream);
 
 /* The two passes of spr_exchange_step_grouped, separately:
- * gsum[g,b,t] = sum_{e in g} scal[b,e] * prev[rowid[e], b, t - lag[e]]. */
+ * gsum[g,b,t] = sum_{e in g} scal[b,e] * prev[rowid[e], b, t - lag[e]].
+ * Replaces, with a spread, one order of _energy_exchange,
+ * RadiosityFast.py:1121-1144. */
 int spr_exchange_gather_grouped(int64_t P, int64_t B, int64_t T, int64_t Vd,
                                 const int64_t *row_ptr,
                                 const int64_t *grp_ptr, const int32_t *rowid,
@@ -255,7 +263,9 @@ int spr_exchange_gather_grouped(int64_t P, int64_t B, int64_t T, int64_t Vd,
                                 const float *prev, float *gsum, void *stream);
 
 /* cur[j,:,b,t] = sum_{g of j} rtab[grp_key[g], b, :] * gsum[g,b,t] and the
- * compensated running total tot_hi + tot_lo += cur. */
+ * compensated running total tot_hi + tot_lo += cur.
+ * Replaces, with a gather, one order of _energy_exchange,
+ * RadiosityFast.py:1121-1144. */
 int spr_exchange_spread_grouped(int64_t P, int64_t D, int64_t B, int64_t T,
                                 const int64_t *grp_ptr,
                                 const int32_t *grp_key, const double *rtab,
@@ -266,7 +276,9 @@ int spr_exchange_spread_grouped(int64_t P, int64_t D, int64_t B, int64_t T,
  * at once: tot_hi + tot_lo += prev + cur, where prev (the previous order,
  * not yet in the total; its own spread ran without the total) must not
  * alias cur.  Reading and writing the total every other order instead of
- * every order saves 12 bytes per state element per two orders. */
+ * every order saves 12 bytes per state element per two orders.
+ * Replaces, with a gather, one order of _energy_exchange,
+ * RadiosityFast.py:1121-1144 (two orders' totals). */
 int spr_exchange_spread_grouped_pair(int64_t P, int64_t D, int64_t B,
                                      int64_t T, const int64_t *grp_ptr,
                                      const int32_t *grp_key,
@@ -280,21 +292,25 @@ int spr_exchange_spread_grouped_pair(int64_t P, int64_t D, int64_t B,
  * cur[j,d] = gsum[diffuse group of j] + gsum[group of j with key d]
  * instead of D multiply-adds per group.  prev, tot_hi, tot_lo as in the
  * two grouped spreads (all NULL: histogram only; prev NULL: add cur to the
- * total; else add prev + cur). */
+ * total; else add prev + cur).
+ * Replaces, with spr_exchange_gather_ds, one order of _energy_exchange,
+ * RadiosityFast.py:1121-1144. */
 int spr_exchange_spread_ds(int64_t P, int64_t D, int64_t B, int64_t T,
                            const int64_t *grp_ptr, const int32_t *grp_key,
                            const float *gsum, const float *prev, float *cur,
                            float *tot_hi, float *tot_lo, void *stream);
 
 /* tot_hi + tot_lo += x[n] (compensated, float64 arithmetic): adds an order
- * left out of the total by the pairing above. */
+ * left out of the total by the pairing above.
+ * The running total of _energy_exchange (RadiosityFast.py:1144). */
 int spr_exchange_total_add(int64_t n, const float *x, float *tot_hi,
                            float *tot_lo, void *stream);
 
 /* One reflection order on a grouped plan (same result as
  * spr_exchange_step on the equivalent generic plan): a gather pass writes
  * the per-group sums gsum[G,B,T] (float32 workspace), a spread pass turns
- * them into cur and the running total. */
+ * them into cur and the running total.
+ * Replaces one order of _energy_exchange, RadiosityFast.py:1121-1144. */
 int spr_exchange_step_grouped(int64_t P, int64_t D, int64_t B, int64_t T,
                               int64_t Vd, const int64_t *row_ptr,
                               const int64_t *grp_ptr, const int32_t *grp_key,
@@ -319,7 +335,9 @@ int spr_brdf_diffuse_specular(int64_t NK, int64_t D, int64_t B,
 /* Pass 1 of the plan: like spr_plan_grouped_sort with the entry key
  * spec_dir[bi(w_i)*S + s_ij]; entries without a specular part (key D)
  * form no group, and each non-empty row gets one more group, its diffuse
- * group, last in the row. */
+ * group, last in the row.
+ * Replaces, with spr_plan_ds_fill, the per-pair terms of one order of
+ * _energy_exchange, RadiosityFast.py:1121-1144. */
 int spr_plan_ds_sort(int64_t P, int64_t D, int64_t max_row,
                      const int64_t *row_ptr, int32_t *src,
                      const double *centers, const int64_t *wall_ids,
@@ -334,7 +352,9 @@ int spr_plan_ds_sort(int64_t P, int64_t D, int64_t max_row,
  * row of ones), to be used with spr_exchange_gather_ds and
  * spr_exchange_spread_grouped.  Optional (may be NULL): krow[Vd] (int32,
  * the BRDF row k of each entry) and wgt[B,Vd] (float64, the pair factor
- * F~_ij exp(-air_b |u|) without the BRDF), which the adjoint uses. */
+ * F~_ij exp(-air_b |u|) without the BRDF), which the adjoint uses.
+ * Replaces, with spr_plan_ds_sort, the per-pair terms of one order of
+ * _energy_exchange, RadiosityFast.py:1121-1144. */
 int spr_plan_ds_fill(int64_t P, int64_t D, int64_t B, double c, double dt,
                      int64_t Vd, const int64_t *row_ptr, const int32_t *src,
                      const int32_t *key, const int64_t *grp_ptr,
@@ -349,7 +369,9 @@ int spr_plan_ds_fill(int64_t P, int64_t D, int64_t B, double c, double dt,
 
 /* Gather pass of the diffuse + specular plan: gsum rows of the specular
  * groups as spr_exchange_gather_grouped with scal, and the row's diffuse
- * group = sum over all its entries with dscal. */
+ * group = sum over all its entries with dscal.
+ * Replaces, with spr_exchange_spread_ds, one order of _energy_exchange,
+ * RadiosityFast.py:1121-1144. */
 int spr_exchange_gather_ds(int64_t P, int64_t B, int64_t T, int64_t Vd,
                            const int64_t *row_ptr, const int64_t *grp_ptr,
                            const int32_t *rowid, const int32_t *lag,
@@ -376,7 +398,9 @@ int spr_plan_tile_order(int64_t P, int64_t JT, int64_t Vd, int64_t G,
 /* Pass 2: mrow[M] (int32 source row), mlag[M,JT] (int32 delay per slot,
  * 0x3fffffff for a slot without that entry, bit 30 set where the slot's
  * group ends: on the last merged entry of the key run) and mw[B,M,JT]
- * (float64 weight per band and slot, 0 for a slot without the entry). */
+ * (float64 weight per band and slot, 0 for a slot without the entry).
+ * Replaces, with spr_plan_tile_order, the per-pair terms of one order of
+ * _energy_exchange, RadiosityFast.py:1121-1144. */
 int spr_plan_tile_fill(int64_t P, int64_t JT, int64_t B, int64_t Vd,
                        int64_t M, const int64_t *row_ptr,
                        const int32_t *rowid, const int32_t *lag,
@@ -386,18 +410,22 @@ int spr_plan_tile_fill(int64_t P, int64_t JT, int64_t B, int64_t Vd,
 
 /* Gather pass on the tiles: gsum bit-identical to
  * spr_exchange_gather_grouped (each slot sums its groups in the same
- * order), for use with spr_exchange_spread_grouped. */
+ * order), for use with spr_exchange_spread_grouped.
+ * Replaces, with a spread, one order of _energy_exchange,
+ * RadiosityFast.py:1121-1144. */
 int spr_exchange_gather_tiled(int64_t P, int64_t JT, int64_t B, int64_t T,
                               int64_t M, const int64_t *tptr,
                               const int64_t *grp_ptr, const int32_t *mrow,
                               const int32_t *mlag, const double *mw,
                               const float *prev, float *gsum, void *stream);
 
-/* etc[P,D,B,T] (float32) = tot_hi + tot_lo (may alias tot_hi). */
+/* etc[P,D,B,T] (float32) = tot_hi + tot_lo (may alias tot_hi).
+ * The total histogram _energy_exchange returns (RadiosityFast.py:1114-1145). */
 int spr_exchange_finalize(int64_t n, const float *tot_hi,
                           const float *tot_lo, float *etc, void *stream);
 
-/* Bytes of workspace spr_energy_exchange needs. */
+/* Bytes of workspace spr_energy_exchange needs.
+ * Sizing helper for spr_energy_exchange; no counterpart in the reference. */
 int64_t spr_energy_exchange_workspace(int64_t P, int64_t D, int64_t B,
                                       int64_t T, int64_t V);
 
@@ -449,7 +477,8 @@ int spr_collect(int64_t NR, int64_t P, int64_t D, int64_t B, int64_t T,
  * ------------------------------------------------------------------- */
 
 /* Plan weights from tables: scal[b,e] = wgt[b,e] cs[krow_e, b],
- * dscal[b,e] = wgt[b,e] a[krow_e, b] (krow, wgt from spr_plan_ds_fill). */
+ * dscal[b,e] = wgt[b,e] a[krow_e, b] (krow, wgt from spr_plan_ds_fill).
+ * Adjoint support; no counterpart in the reference. */
 int spr_ds_weights(int64_t B, int64_t Vd, const int32_t *krow,
                    const double *wgt, const double *a, const double *cs,
                    double *scal, double *dscal, void *stream);
@@ -467,7 +496,8 @@ int spr_ds_injection(int64_t P, int64_t D, int64_t B, const double *source,
                      void *stream);
 
 /* Source-major view of a diffuse + specular plan.  Pass 1: trow_ptr[P+1]
- * (int64) = exclusive scan of the entries per source (rowid / D). */
+ * (int64) = exclusive scan of the entries per source (rowid / D).
+ * Adjoint support; no counterpart in the reference. */
 int spr_plan_ds_transpose_rows(int64_t P, int64_t D, const int64_t *row_ptr,
                                const int32_t *rowid, int64_t *trow_ptr,
                                void *stream);
@@ -479,7 +509,8 @@ int spr_plan_ds_transpose_rows(int64_t P, int64_t D, const int64_t *row_ptr,
  * the entry, 0 if none), tgptr[P*D+1] (int64: first entry of source i with
  * direction >= d at i*D + d), tkey[Vd] (int32: direction << 25 |
  * destination).  cursor[P] (int64) is workspace; key[Vd] is the plan's
- * entry key (specular direction or D). */
+ * entry key (specular direction or D).
+ * Adjoint support; no counterpart in the reference. */
 int spr_plan_ds_transpose_fill(int64_t P, int64_t D, int64_t Vd,
                                int64_t max_trow, const int64_t *row_ptr,
                                const int32_t *rowid, const int32_t *lag,
@@ -488,21 +519,24 @@ int spr_plan_ds_transpose_fill(int64_t P, int64_t D, int64_t Vd,
                                int32_t *tj, int32_t *tlag, int32_t *tm,
                                int64_t *tgptr, void *stream);
 
-/* Transposed weights: twa[b,q] = dscal[b, te[q]], twc[b,q] = scal[b,te[q]]. */
+/* Transposed weights: twa[b,q] = dscal[b, te[q]], twc[b,q] = scal[b,te[q]].
+ * Adjoint support; no counterpart in the reference. */
 int spr_ds_transposed_weights(int64_t B, int64_t Vd, const int32_t *te,
                               const double *scal, const double *dscal,
                               double *twa, double *twc, void *stream);
 
 /* Adjoint of spr_collect (mono): lam[P,D,B,T] (float32, zero-filled here)
  * = dL/d(total histogram) for grad[NR,B,T] = dL/d(receiver histograms);
- * the other arguments as for spr_collect. */
+ * the other arguments as for spr_collect.
+ * Adjoint of RadiosityFast.py:1148-1185; no counterpart in the reference. */
 int spr_collect_adjoint(int64_t NR, int64_t P, int64_t D, int64_t B,
                         int64_t T, const float *grad, const double *points,
                         const double *centers, const int64_t *idx,
                         const double *factor, const double *air, double c,
                         double dt, float *lam, void *stream);
 
-/* musum[P,B,T] = sum over d of mu[P,D,B,T] (float64 sum). */
+/* musum[P,B,T] = sum over d of mu[P,D,B,T] (float64 sum).
+ * Adjoint support; no counterpart in the reference. */
 int spr_direction_sum(int64_t P, int64_t D, int64_t B, int64_t T,
                       const float *mu, float *musum, void *stream);
 
@@ -513,7 +547,8 @@ int spr_direction_sum(int64_t P, int64_t D, int64_t B, int64_t T,
  * products to ga[2,B,Vd] (float64): ga[0,b,q] += sum_t musum[j,b,t+n]
  * ek[i,d,b,t], ga[1,b,q] += sum_t munext[j,m,b,t+n] ek[i,d,b,t].  tkey is
  * spr_plan_ds_transpose_fill's tkey (direction << 25 | destination), kept
- * after the fill.  muout must not alias an input.  T < 2^29, P < 2^25. */
+ * after the fill.  muout must not alias an input.  T < 2^29, P < 2^25.
+ * Adjoint of RadiosityFast.py:1121-1144; no counterpart in the reference. */
 int spr_exchange_adjoint_ds(int64_t P, int64_t D, int64_t B, int64_t T,
                             int64_t Vd, const int64_t *tgptr,
                             const int32_t *tkey, const int32_t *tlag,
@@ -524,7 +559,8 @@ int spr_exchange_adjoint_ds(int64_t P, int64_t D, int64_t B, int64_t T,
                             void *stream);
 
 /* Table gradients from ga: gA[k,b] += wgt[b,e] ga[0,b,q] and gC[k,b] +=
- * wgt[b,e] ga[1,b,q] for e = te[q], k = krow[e] (float64 atomics). */
+ * wgt[b,e] ga[1,b,q] for e = te[q], k = krow[e] (float64 atomics).
+ * Adjoint support; no counterpart in the reference. */
 int spr_ds_table_grad(int64_t B, int64_t Vd, const int32_t *te,
                       const int32_t *krow, const double *wgt,
                       const double *ga, double *gA, double *gC,
@@ -532,7 +568,8 @@ int spr_ds_table_grad(int64_t B, int64_t Vd, const int32_t *te,
 
 /* Injection part of the table gradients, from mu0 = dL/dE_0[P,D,B,T]:
  * gA[k0_p,b] += energy0[p,b] sum_d mu0[p,d,b,n_p], gC[k0_p,b] +=
- * energy0[p,b] mu0[p,m,b,n_p] with n_p = int(d0[p]/c/dt). */
+ * energy0[p,b] mu0[p,m,b,n_p] with n_p = int(d0[p]/c/dt).
+ * Adjoint of RadiosityFast.py:988-1034; no counterpart in the reference. */
 int spr_ds_injection_grad(int64_t P, int64_t D, int64_t B, int64_t T,
                           const double *energy0, const double *d0, double c,
                           double dt, const int32_t *k0,
