@@ -83,15 +83,26 @@ __global__ void ds_injection_kernel(
   }
 }
 
-// musum[j,b,t] = sum_d mu[j,d,b,t] (float64 sum, float32 result)
+// musum[j,b,t] = sum_d mu[j,d,b,t] (float64 sum in direction order,
+// float32 result); 8 directions' loads are issued before they are added.
 __global__ void direction_sum_kernel(int64_t P, int64_t D, int64_t B,
                                      int64_t T, const float *__restrict__ mu,
                                      float *__restrict__ musum) {
   const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= P * B * T) return;
   const int64_t t = q % T, b = (q / T) % B, j = q / (B * T);
+  const float *p = mu + (j * D * B + b) * T + t;  // element (j, 0, b, t)
+  const int64_t stride = B * T;                  // next direction
   double s = 0.0;
-  for (int64_t d = 0; d < D; ++d) s += (double)mu[((j * D + d) * B + b) * T + t];
+  int64_t d = 0;
+  for (; d + 8 <= D; d += 8) {
+    float v[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) v[k] = p[(d + k) * stride];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) s += (double)v[k];
+  }
+  for (; d < D; ++d) s += (double)p[d * stride];
   musum[q] = (float)s;
 }
 
