@@ -64,6 +64,10 @@ def _direction_sets(sets):
     return out
 
 
+def _as_array(v):
+    return None if v is None else np.asarray(v)
+
+
 def _as_state_value(v):
     return "None" if v is None else v
 
@@ -133,7 +137,21 @@ class DirectionalRadiosityFast:
         self._bake = None
         self._plan = None
         self._etc = None
+        # saved arrays, validated by check() before they are uploaded
+        self._form_factors_in = _as_array(form_factors)
+        self._form_factors_tilde_in = _as_array(form_factors_tilde)
+        self._distance_patches_to_source_in = _as_array(
+            distance_patches_to_source)
+        self._energy_init_source_in = _as_array(energy_init_source)
+        self._energy_exchange_etc_in = _as_array(energy_exchange_etc)
+        if (energy_init_source is None) != (
+                distance_patches_to_source is None):
+            raise ValueError("energy_init_source and "
+                             "distance_patches_to_source go together")
         self.check()
+        self._form_factors_in = self._form_factors_tilde_in = None
+        self._distance_patches_to_source_in = None
+        self._energy_init_source_in = self._energy_exchange_etc_in = None
         if visibility_matrix is not None:
             self._restore_bake(visibility_matrix, visible_patches,
                                form_factors, form_factors_tilde,
@@ -159,10 +177,29 @@ class DirectionalRadiosityFast:
                                              patch_size, len(polygon_list))
         return cls(walls_points, walls_normal, walls_up_vector, pts, n, ids)
 
+    def _n_outgoing_directions(self):
+        """D of the set BRDF direction sets (1 without BRDFs)."""
+        for c in (self._brdf_outgoing_directions
+                  if self._brdf_outgoing_directions is not None else []):
+            if c is not None:
+                return int(c.cartesian.shape[0])
+        return 1
+
     def check(self):
-        """Shape validation (reference ``RadiosityFast.py:211-337``)."""
+        """Input validation (reference ``RadiosityFast.py:211-337``).
+
+        Raises the reference's ValueErrors for the geometry, frequencies,
+        form factors, BRDF index and direction sets, F~, air attenuation,
+        speed of sound, time resolution, duration and the saved source
+        distances, source energies and histograms.  Walls without a BRDF
+        (``set_wall_brdf`` on a subset of walls) keep ``None`` direction
+        sets, which the reference's type check would refuse; every set
+        wall must have the same number of directions.  The BRDF tables must
+        match the direction sets and bands, and BRDF indices must name a
+        table, since the GPU kernels index them without bounds checks.
+        """
         n_walls = self._walls_points.shape[0]
-        if self._walls_points.shape[2] != 3:
+        if self._walls_points.ndim != 3 or self._walls_points.shape[2] != 3:
             raise ValueError(
                 "Walls need to be of shape (n_walls, n_points, 3)")
         if self._walls_up_vector.shape != (n_walls, 3):
@@ -171,7 +208,8 @@ class DirectionalRadiosityFast:
         if self._walls_normal.shape != (n_walls, 3):
             raise ValueError(
                 "Normal of walls need to be of shape (n_walls, 3)")
-        if (self._patches_points.shape[0] != self.n_patches
+        if (self._patches_points.ndim != 3
+                or self._patches_points.shape[0] != self.n_patches
                 or self._patches_points.shape[2] != 3):
             raise ValueError(
                 "Patches need to be of shape (n_patches, n_points, 3)")
@@ -183,30 +221,154 @@ class DirectionalRadiosityFast:
             raise ValueError(
                 "patch_to_wall_ids does contain other ids than "
                 "range(n_walls)")
+        n_bins = 1
+        if self._frequencies is not None:
+            if self._frequencies.ndim != 1:
+                raise ValueError("Frequencies need to be of shape (n_bins,)")
+            n_bins = self._frequencies.size
+        n_dirs = self._n_outgoing_directions()
+        if self._brdf_index is not None:
+            if self._brdf_index.ndim != 1 or len(self._brdf_index) != n_walls:
+                raise ValueError("brdf_index need to be of shape (n_walls,)")
+        for name, sets in (
+                ("brdf_incoming_directions", self._brdf_incoming_directions),
+                ("brdf_outgoing_directions", self._brdf_outgoing_directions)):
+            if sets is None:
+                continue
+            if len(sets) != n_walls:
+                raise ValueError(f"{name} need to be a list of n_walls "
+                                 "Coordinates")
+            if name == "brdf_outgoing_directions" and any(
+                    c is not None and c.cartesian.shape[0] != n_dirs
+                    for c in sets):
+                raise ValueError(f"{name} need the same number of "
+                                 "directions on every wall")
+        if self._brdf is not None:
+            n_in = None
+            if self._brdf_incoming_directions is not None:
+                n_in = {c.cartesian.shape[0]
+                        for c in self._brdf_incoming_directions
+                        if c is not None}
+            # the kernels read each table as flat [S, D, B] memory
+            sizes = ({k * n_dirs * n_bins for k in n_in} if n_in else None)
+            for b in self._brdf:
+                if (b.size % (n_dirs * n_bins) != 0 or b.size == 0
+                        or (sizes and b.size not in sizes)):
+                    raise ValueError(
+                        "brdf need to be a list of arrays of shape "
+                        "(n_incoming_directions, n_outgoing_directions, "
+                        "n_bins)")
+            if self._brdf_index is not None and (
+                    (self._brdf_index >= len(self._brdf)).any()
+                    or (self._brdf_index < -1).any()):
+                raise ValueError("brdf_index must name an entry of brdf "
+                                 "(or -1 for a wall without a BRDF)")
+        if self._form_factors_in is not None:
+            if self._form_factors_in.shape != (self.n_patches,
+                                               self.n_patches):
+                raise ValueError(
+                    "form_factors need to be of shape (n_patches, n_patches)")
+        if self._form_factors_tilde_in is not None:
+            if self._form_factors_tilde_in.shape != (
+                    self.n_patches, self.n_patches, n_dirs, n_bins):
+                raise ValueError(
+                    "form_factors_tilde need to be of shape "
+                    "(n_patches, n_patches, n_outgoing_directions, n_bins)")
+        if self._air_attenuation is not None:
+            if (self._air_attenuation.ndim != 1
+                    or self._air_attenuation.shape[0] != n_bins):
+                raise ValueError(
+                    "Air attenuation need to be of shape (n_bins,)")
+        if self._speed_of_sound is not None and self._speed_of_sound <= 0:
+            raise ValueError("Speed of sound must be positive and non-zero")
+        if (self._etc_time_resolution is not None
+                and self._etc_time_resolution <= 0):
+            raise ValueError("Time resolution must be positive and non-zero")
+        if self._etc_duration is not None and self._etc_duration <= 0:
+            raise ValueError("Duration must be positive and non-zero")
+        if self._distance_patches_to_source_in is not None:
+            if self._distance_patches_to_source_in.shape != (self.n_patches,):
+                raise ValueError(
+                    "distance_patches_to_source need to be of shape "
+                    "(n_patches,)")
+        if self._energy_init_source_in is not None:
+            if self._energy_init_source_in.shape != (
+                    self.n_patches, n_dirs, n_bins):
+                raise ValueError(
+                    "energy_init_source need to be of shape "
+                    "(n_patches, n_outgoing_directions, n_bins)")
+        if self._energy_exchange_etc_in is not None:
+            if self._etc_duration is None or \
+                    self._etc_time_resolution is None:
+                raise ValueError(
+                    "energy_exchange_etc needs etc_duration and "
+                    "etc_time_resolution")
+            n_samples = int(self._etc_duration / self._etc_time_resolution)
+            if self._energy_exchange_etc_in.shape != (
+                    self.n_patches, n_dirs, n_bins, n_samples):
+                raise ValueError(
+                    "energy_exchange_etc need to be of shape "
+                    "(n_patches, n_outgoing_directions, n_bins, n_samples)")
 
     def _restore_bake(self, vis, pairs, ff, fft_dense, p2o):
         """Upload a saved bake (the reference's visibility matrix, visible
-        pair list, form factors, direction map and, optionally, F~)."""
+        pair list, form factors, direction map and, optionally, F~).
+
+        Every array the kernels index is checked first: shapes against
+        n_patches, the direction count and the bands, pair indices within
+        the patches and above the diagonal, direction indices within
+        [0, D] (D marks pairs that see each other in neither orientation).
+        """
         P = self.n_patches
-        vis = np.asarray(vis, dtype=bool)
+        vis = np.asarray(vis)
         if vis.shape != (P, P):
             raise ValueError("visibility_matrix must be (n_patches, "
                              "n_patches)")
+        vis = vis.astype(bool)
+        if ff is None or np.asarray(ff).shape != (P, P):
+            raise ValueError(
+                "form_factors need to be of shape (n_patches, n_patches)")
         if self._brdf_outgoing_directions is not None:
             brdf, bidx, sources, receivers = self._brdf_tables()
             D = receivers.shape[1]
         else:
             brdf = bidx = sources = receivers = None
             D = 1
+        n_bins = 1 if self._frequencies is None else self.n_bins
+        if pairs is not None:
+            pairs = np.asarray(pairs).reshape(-1, 2) if np.size(pairs) else \
+                np.zeros((0, 2), np.int64)
+            if ((pairs < 0).any() or (pairs >= P).any()
+                    or (pairs[:, 0] >= pairs[:, 1]).any()):
+                raise ValueError("visible_patches need to be pairs (i, j) "
+                                 "of patch indices with i < j")
+            if not vis[pairs[:, 0], pairs[:, 1]].all() or \
+                    len(pairs) != int(np.triu(vis, 1).sum()):
+                raise ValueError("visible_patches must list the visible "
+                                 "pairs of visibility_matrix")
+        if p2o is not None:
+            p2o = np.asarray(p2o, dtype=np.int64)
+            if p2o.shape != (P, P):
+                raise ValueError("patch_2_brdf_outgoing_index need to be of "
+                                 "shape (n_patches, n_patches)")
+            if (p2o < 0).any() or (p2o > D).any():
+                raise ValueError("patch_2_brdf_outgoing_index values must be "
+                                 "outgoing direction indices (0..D)")
+        if fft_dense is not None and np.asarray(fft_dense).shape != (
+                P, P, D, n_bins):
+            raise ValueError(
+                "form_factors_tilde need to be of shape "
+                "(n_patches, n_patches, n_outgoing_directions, n_bins)")
         self._bake = dict(
             vis=ops.as_dev(vis, torch.uint8),
             pairs=ops.as_dev(np.asarray(pairs, dtype=np.int32).reshape(-1, 2)
                              if pairs is not None else
-                             np.argwhere(vis).astype(np.int32), torch.int32),
+                             np.argwhere(np.triu(vis, 1)).astype(np.int32),
+                             torch.int32),
             ff=ops.as_dev(np.asarray(ff, float), torch.float64),
-            p2o=ops.as_dev(np.asarray(p2o, dtype=np.int64) if p2o is not None
+            p2o=ops.as_dev(p2o if p2o is not None
                            else np.zeros((P, P), np.int64), torch.int64),
-            D=D, n_bins=1 if self._frequencies is None else self.n_bins,
+            D=D, n_bins=n_bins,
             air=None if self._air_attenuation is None else
             self._air_attenuation.copy(),
             brdf=brdf, brdf_index=bidx, sources=sources, receivers=receivers,
@@ -373,7 +535,19 @@ class DirectionalRadiosityFast:
                 "Frequencies do not match"
 
     def _brdf_tables(self):
-        """Stacked BRDF tables and per-wall direction arrays (host)."""
+        """Stacked BRDF tables and per-wall direction arrays (host).
+
+        Every wall needs a BRDF here (the kernels index the tables by
+        wall); the reference fails there too, on its unset walls."""
+        if (any(s is None for s in self._brdf_incoming_directions)
+                or any(s is None for s in self._brdf_outgoing_directions)
+                or (np.asarray(self._brdf_index) < 0).any()):
+            raise ValueError("set_wall_brdf must cover every wall before "
+                             "the bake, the source or the receivers")
+        if len({s.cartesian.shape for s in
+                self._brdf_incoming_directions}) != 1:
+            raise ValueError("brdf_incoming_directions need the same number "
+                             "of directions on every wall")
         sources = np.array([s.cartesian for s in
                             self._brdf_incoming_directions])
         receivers = np.array([s.cartesian for s in
@@ -538,11 +712,27 @@ class DirectionalRadiosityFast:
                                   recalculate=False):
         """Reflection-order energy exchange
         (reference ``RadiosityFast.py:524-568``)."""
+        if speed_of_sound <= 0:
+            raise ValueError("Speed of sound must be positive and non-zero")
+        if etc_time_resolution <= 0:
+            raise ValueError("Time resolution must be positive and non-zero")
+        if etc_duration <= 0:
+            raise ValueError("Duration must be positive and non-zero")
         n_samples = int(etc_duration / etc_time_resolution)
         if self._etc is None or recalculate:
+            if "energy_init_source" not in self._dev:
+                raise RuntimeError("init_source_energy() must run before "
+                                   "the exchange")
             e0d = self._dev["energy_init_source"]
             d0 = self._dev["distance_patches_to_source"]
             P, D, B = e0d.shape
+            if P != self.n_patches or tuple(d0.shape) != (P,):
+                raise ValueError(
+                    "energy_init_source need to be of shape "
+                    "(n_patches, n_outgoing_directions, n_bins)")
+            if n_samples < 1:
+                raise ValueError("etc_duration must hold at least one "
+                                 "etc_time_resolution")
             if max_reflection_order >= 1 and self._bake is None:
                 raise RuntimeError("bake_geometry() must run before an "
                                    "exchange with reflections")
