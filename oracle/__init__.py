@@ -58,6 +58,23 @@ def _load():
                                       _u8p, _int]),
         "orc_point_visibility": (None, [_f64p, _i64, _f64p, _i64, _f64p,
                                         _f64p, _int, _u8p]),
+        "orc_p2p_visibility_pairs": (None, [_f64p, _i64, _f64p, _f64p, _int,
+                                            _i64, _i64p, _u8p, _int]),
+        "orc_fft_pairs": (None, [_i64, _i64p, _f64p, _i64, _f64p, _f64p,
+                                 ctypes.c_void_p, _i64p, _i64, _i64,
+                                 ctypes.c_void_p, ctypes.c_void_p,
+                                 ctypes.c_void_p, _f64p]),
+        "orc_exchange_step": (None, [_i64, _i64, _i64, _i64, _f64p, _f64p,
+                                     _f64p, _f64p, _i64p, _dbl, _dbl, _u8p,
+                                     _int]),
+        "orc_exchange_adjoint_step": (None, [_i64, _i64, _i64, _i64, _f64p,
+                                             _f64p, _f64p, _f64p, _f64p,
+                                             _i64p, _dbl, _dbl, _u8p, _f64p,
+                                             _f64p, _int]),
+        "orc_source_index_pairs": (None, [_i64, _i64p, _f64p, _i64, _f64p,
+                                          _i64p, _i64p]),
+        "orc_exchange_rows": (None, [_i64, _i64, _i64, _i64, _i64p, _i64p,
+                                     _i64p, _f64p, _f64p, _f64p, _int]),
         "orc_stokes": (_dbl, [_f64p, _f64p, _dbl]),
         "orc_nusselt": (_dbl, [_f64p, _f64p, _f64p, _f64p, _int]),
         "orc_universal_ff": (_dbl, [_f64p, _f64p, _dbl, _f64p, _f64p]),
@@ -133,6 +150,23 @@ def p2p_visibility(centers, surf_normals, surf_points, nthreads=None):
     _load().orc_p2p_visibility(P, centers, sp.shape[0], _c(surf_normals), sp,
                                sp.shape[1], out,
                                nthreads or default_threads())
+    return out.astype(bool)
+
+
+def p2p_visibility_pairs(centers, surf_normals, surf_points, pairs,
+                         nthreads=None):
+    """geometry.py:750-797 for a list of pairs (i < j each): the
+    visibility of each, as ``p2p_visibility(...)[i, j]`` (for sampled rows
+    of scenes too large for the full matrix)."""
+    centers = _c(centers)
+    sp = _c(surf_points)
+    pairs = _c(pairs, np.int64).reshape(-1, 2)
+    if (pairs[:, 0] >= pairs[:, 1]).any():
+        raise ValueError("pairs must be (i, j) with i < j")
+    out = np.zeros(pairs.shape[0], dtype=np.uint8)
+    _load().orc_p2p_visibility_pairs(centers, sp.shape[0], _c(surf_normals),
+                                     sp, sp.shape[1], pairs.shape[0], pairs,
+                                     out, nthreads or default_threads())
     return out.astype(bool)
 
 
@@ -237,6 +271,29 @@ def fft(vis, ff, n_bins, centers, areas, air, wall_ids, brdf, brdf_index,
     return out
 
 
+def fft_pairs(pairs, ff_upper, n_bins, centers, areas, air, wall_ids, brdf,
+              brdf_index, sources, n_directions):
+    """RadiosityFast.py:1234-1272 for directed visible pairs (i -> j):
+    ``fft(...)[i, j]`` for each, with ff_upper[q] = ff[min(i,j), max(i,j)].
+    Returns (n, D, B)."""
+    pairs = _c(pairs, np.int64).reshape(-1, 2)
+    if brdf is None:
+        S, D = 1, 1
+        brdf_c = idx_c = src_c = None
+    else:
+        brdf_c = _c(brdf)
+        S, D = brdf_c.shape[1], int(n_directions)
+        idx_c = _c(brdf_index, np.int64)
+        src_c = _c(sources)
+    out = np.zeros((pairs.shape[0], D, n_bins))
+    air_c = None if air is None else _c(air)
+    _load().orc_fft_pairs(pairs.shape[0], pairs, _c(ff_upper), n_bins,
+                          _c(centers), _c(areas), _ptr(air_c),
+                          _c(wall_ids, np.int64), S, D, _ptr(brdf_c),
+                          _ptr(idx_c), _ptr(src_c), out)
+    return out
+
+
 def add_directional(energy0, source, centers, wall_ids, sources, receivers,
                     brdf, brdf_index):
     energy0 = _c(energy0)
@@ -273,6 +330,27 @@ def exchange(n_samples, e0dir, d0, dist, fft_, p2o, c, dt, max_order, pairs,
                             _c(fft_), _c(p2o, np.int64), c, dt, max_order,
                             pairs.shape[0], pairs, out, work, nthreads):
         raise IndexError("source delay exceeds the histogram length")
+    return out
+
+
+def exchange_rows(T, row_ptr, row_src, row_lag, coef, prev_rows,
+                  nthreads=None):
+    """One order (RadiosityFast.py:1121-1143) for destinations whose entries
+    (row_ptr CSR, in accumulation order) read compact source rows
+    prev_rows[row_src[e]] ([n_rows, B, T]) delayed by row_lag[e] with
+    coefficients coef[e] ([D, B]).  Returns cur [n_dest, D, B, T]."""
+    row_ptr = _c(row_ptr, np.int64)
+    coef = _c(coef)
+    prev_rows = _c(prev_rows)
+    n_dest = row_ptr.shape[0] - 1
+    _, D, B = coef.shape if coef.ndim == 3 else (0, 1, prev_rows.shape[1])
+    if prev_rows.shape[1:] != (B, T):
+        raise ValueError("prev_rows must be [n_rows, B, T]")
+    out = np.zeros((n_dest, D, B, T))
+    _load().orc_exchange_rows(T, D, B, n_dest, row_ptr,
+                              _c(row_src, np.int64), _c(row_lag, np.int64),
+                              coef, prev_rows, out,
+                              nthreads or default_threads())
     return out
 
 

@@ -4,8 +4,9 @@ tables and absorption/scattering coefficients.
 TEST INFRASTRUCTURE ONLY (see ``oracle/__init__.py``).
 
 The reference has no gradients.  This is the plain reverse-mode
-derivative of its forward path restated in numpy on the dense arrays the
-reference itself uses (``F~ [P,P,D,B]``, ``E [P,D,B,T]``):
+derivative of its forward path on the dense arrays the reference itself
+uses (``F~ [P,P,D,B]``, ``E [P,D,B,T]``), numpy glue around the C oracle's
+per-order loops (``orc_exchange_step``, ``orc_exchange_adjoint_step``):
 
 * exchange orders E_0..E_K by the loop of ``_energy_exchange``
   (``RadiosityFast.py:1114-1143``: per visible pair, both orientations,
@@ -21,7 +22,9 @@ reference itself uses (``F~ [P,P,D,B]``, ``E [P,D,B,T]``):
   ``create_from_scattering`` (``brdf.py:104-118``) to (alpha, s).
 
 Pinned by central finite differences of the FP64 oracle forward
-(``tests/test_oracle_adjoint.py``).  Sized for small scenes only.
+(``tests/test_oracle_adjoint.py``).  Dense arrays bound the size: the
+1,072-patch shoebox with 64 directions and 6 bands fits at a few hundred
+bins (F~ and its gradient 3.5 GB each).
 """
 from __future__ import annotations
 
@@ -30,23 +33,31 @@ import numpy as np
 import oracle as orc
 
 
-def _pairs_both(pairs):
-    for i, j in pairs:
-        yield int(i), int(j)
-        yield int(j), int(i)
+def _vis(P, pairs):
+    """Upper-triangular visibility (uint8 [P, P]) of a visible-pair list."""
+    vis = np.zeros((P, P), np.uint8)
+    pairs = np.asarray(pairs, np.int64).reshape(-1, 2)
+    vis[pairs[:, 0], pairs[:, 1]] = 1
+    return vis
 
 
 def exchange_orders(T, e0dir, d0, dist, fft, p2o, c, dt, K, pairs):
-    """[E_0, ..., E_K] in FP64 (RadiosityFast.py:1037-1143 per order)."""
+    """[E_0, ..., E_K] in FP64 (RadiosityFast.py:1037-1143 per order;
+    each order by ``orc_exchange_step``: per destination, every visible
+    source in ascending order, ``E_{k+1}[j, :, :, n:] += F~[i,j,:,:,None] *
+    E_k[i, p2o[i,j], :, :T-n]``)."""
+    e0dir = np.ascontiguousarray(e0dir, float)
+    P, D, B = e0dir.shape
+    vis = _vis(P, pairs)
+    lib = orc._load()
+    dist = np.ascontiguousarray(dist, float)
+    fft = np.ascontiguousarray(fft, float)
+    p2o = np.ascontiguousarray(p2o, np.int64)
     E = [orc.init_energy(T, e0dir, d0, c, dt)]
     for _ in range(K):
-        prev, cur = E[-1], np.zeros_like(E[-1])
-        for i, j in _pairs_both(pairs):
-            n = orc.delay_floor(dist[i, j], c, dt)
-            if n >= T:
-                continue
-            d = p2o[i, j]
-            cur[j, :, :, n:] += fft[i, j, :, :, None] * prev[i, d, :, :T - n]
+        cur = np.empty_like(E[-1])
+        lib.orc_exchange_step(T, P, D, B, E[-1], cur, dist, fft, p2o, c, dt,
+                              vis, orc.default_threads())
         E.append(cur)
     return E
 
@@ -75,21 +86,29 @@ def collect_adjoint(grad, receivers, r):
 
 
 def exchange_adjoint(lam, E, dist, fft, p2o, c, dt, pairs):
-    """(dL/dF~ [P,P,D,B], dL/dE_0 [P,D,B,T]) for dL/dtotal = lam."""
-    T = lam.shape[-1]
+    """(dL/dF~ [P,P,D,B], dL/dE_0 [P,D,B,T]) for dL/dtotal = lam.
+
+    Order by order from the last (``orc_exchange_adjoint_step``): with
+    mu_K = lam, mu_k = lam + A^T mu_{k+1}, where for every directed visible
+    pair i -> j (delay n < T, d = p2o[i,j])
+    ``mu_k[i, d, :, :T-n] += sum_e F~[i,j,e,:] mu_{k+1}[j, e, :, n:]`` and
+    ``dL/dF~[i,j,e,:] += sum_t mu_{k+1}[j,e,:,t+n] E_k[i,d,:,t]``."""
+    lam = np.ascontiguousarray(lam, float)
+    P, D, B, T = lam.shape
     K = len(E) - 1
+    vis = _vis(P, pairs)
+    lib = orc._load()
+    dist = np.ascontiguousarray(dist, float)
+    fft = np.ascontiguousarray(fft, float)
+    p2o = np.ascontiguousarray(p2o, np.int64)
     mu = lam.copy()                                        # mu_K
     dfft = np.zeros_like(fft)
     for k in range(K - 1, -1, -1):
-        new = lam.copy()
-        for i, j in _pairs_both(pairs):
-            n = orc.delay_floor(dist[i, j], c, dt)
-            if n >= T:
-                continue
-            d = p2o[i, j]
-            mj = mu[j, :, :, n:]                           # [D,B,T-n]
-            new[i, d, :, :T - n] += np.einsum("db,dbt->bt", fft[i, j], mj)
-            dfft[i, j] += np.einsum("dbt,bt->db", mj, E[k][i, d, :, :T - n])
+        new = np.empty_like(lam)
+        lib.orc_exchange_adjoint_step(T, P, D, B, lam, mu,
+                                      np.ascontiguousarray(E[k]), dist, fft,
+                                      p2o, c, dt, vis, new, dfft,
+                                      orc.default_threads())
         mu = new
     return dfft, mu
 
@@ -104,11 +123,16 @@ def table_gradient(r, dfft, mu0, tables_shape):
                 r.patches_area, r.air, r.patch_to_wall_ids, None, None,
                 None, None)[:, :, 0, :]                    # [P,P,B]
     gt = np.zeros(tables_shape)
-    for i, j in _pairs_both(r.visible_patches):
-        wi = r.patch_to_wall_ids[i]
-        k = r.brdf_index[wi] * S + orc.source_index(
-            centers[i], centers[j], r.brdf_in, wi)
-        gt[k] += dfft[i, j] * w[i, j][None, :]
+    vp = np.asarray(r.visible_patches, np.int64).reshape(-1, 2)
+    both = np.concatenate([vp, vp[:, ::-1]])               # i -> j, j -> i
+    ii, jj = both[:, 0], both[:, 1]
+    s_idx = np.zeros(len(both), np.int64)
+    wall = np.ascontiguousarray(r.patch_to_wall_ids, np.int64)
+    orc._load().orc_source_index_pairs(
+        len(both), np.ascontiguousarray(both), np.ascontiguousarray(centers),
+        S, np.ascontiguousarray(r.brdf_in, float), wall, s_idx)
+    k = np.asarray(r.brdf_index)[wall[ii]] * S + s_idx
+    np.add.at(gt, k, dfft[ii, jj] * w[ii, jj][:, None, :])
     d0 = r.distance_patches_to_source
     for i in range(P):
         wi = r.patch_to_wall_ids[i]

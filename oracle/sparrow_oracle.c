@@ -231,6 +231,32 @@ void orc_p2p_visibility(int64_t P, const double *centers, int64_t NS,
   }
 }
 
+/* geometry.py:750-797 _check_patch2patch_visibility, the loop body for a
+ * list of pairs (i < j each, int64 [n,2]): the same AND over all surfaces
+ * as orc_p2p_visibility, for sampled rows of large scenes. */
+void orc_p2p_visibility_pairs(const double *centers, int64_t NS,
+                              const double *surf_normals,
+                              const double *surf_pts, int npts, int64_t n,
+                              const int64_t *pairs, uint8_t *out,
+                              int nthreads) {
+#ifdef _OPENMP
+  if (nthreads < 1) nthreads = 1;
+#pragma omp parallel for schedule(dynamic, 64) num_threads(nthreads)
+#endif
+  for (int64_t q = 0; q < n; ++q) {
+    int64_t i = pairs[2 * q], j = pairs[2 * q + 1];
+    int vis = 1;
+    int64_t k = 0;
+    while (vis && k != NS) {
+      vis = orc_basic_visibility(centers + 3 * i, centers + 3 * j,
+                                 surf_pts + (size_t)k * npts * 3, npts,
+                                 surf_normals + 3 * k);
+      k += 1;
+    }
+    out[q] = (uint8_t)vis;
+  }
+}
+
 /* geometry.py:799-839 _check_point2patch_visibility */
 void orc_point_visibility(const double *point, int64_t P,
                           const double *centers, int64_t NS,
@@ -676,6 +702,35 @@ int64_t orc_source_index(const double *pos_h, const double *pos_i, int64_t S,
   return nearest_direction(sources + (size_t)wall * S * 3, S, d);
 }
 
+/* RadiosityFast.py:1250-1270, the body of _form_factors_with_directivity_dim
+ * for one directed visible pair i -> j: f = ff[i,j] (i < j) or
+ * ff[j,i] A_j / A_i (given), times exp(-air_b |unit(c_i - c_j)|), times
+ * brdf[bi(wall_i), s*, :, :].  o: [D, B]. */
+static void fft_pair(int64_t i, int64_t j, double f, int64_t B,
+                     const double *centers, const double *air,
+                     const int64_t *wall_ids, int64_t S, int64_t D,
+                     const double *brdf, const int64_t *brdf_index,
+                     const double *sources, double *o) {
+  size_t blk = (size_t)D * B;
+  double diff[3];
+  sub3(centers + 3 * i, centers + 3 * j, diff);
+  double nd = norm3(diff);
+  for (int c = 0; c < 3; ++c) diff[c] = diff[c] / nd;
+  double distance = norm3(diff);
+  for (size_t k = 0; k < blk; ++k) o[k] = f;
+  if (air)
+    for (int64_t d = 0; d < D; ++d)
+      for (int64_t b = 0; b < B; ++b)
+        o[d * B + b] = o[d * B + b] * exp(-air[b] * distance);
+  if (brdf) {
+    int64_t w = wall_ids[i];
+    int64_t s = orc_source_index(centers + 3 * i, centers + 3 * j, S,
+                                 sources, w);
+    const double *r = brdf + ((size_t)brdf_index[w] * S + s) * blk;
+    for (size_t k = 0; k < blk; ++k) o[k] = o[k] * r[k];
+  }
+}
+
 /* RadiosityFast.py:1234-1272 _form_factors_with_directivity_dim */
 void orc_fft(int64_t P, const uint8_t *vis, const double *ff, int64_t B,
              const double *centers, const double *areas, const double *air,
@@ -693,25 +748,26 @@ void orc_fft(int64_t P, const uint8_t *vis, const double *ff, int64_t B,
     int64_t j = (ii / P) % P;
     int visible = (i < j) ? vis[i * P + j] : vis[j * P + i];
     if (!visible) continue;
-    double diff[3];
-    sub3(centers + 3 * i, centers + 3 * j, diff);
-    double nd = norm3(diff);
-    for (int c = 0; c < 3; ++c) diff[c] = diff[c] / nd;
     double f = (i < j) ? ff[i * P + j] : (ff[j * P + i] * areas[j]) / areas[i];
-    double distance = norm3(diff);
-    double *o = out + ((size_t)i * P + j) * blk;
-    for (size_t k = 0; k < blk; ++k) o[k] = f;
-    if (air)
-      for (int64_t d = 0; d < D; ++d)
-        for (int64_t b = 0; b < B; ++b)
-          o[d * B + b] = o[d * B + b] * exp(-air[b] * distance);
-    if (brdf) {
-      int64_t w = wall_ids[i];
-      int64_t s = orc_source_index(centers + 3 * i, centers + 3 * j, S,
-                                   sources, w);
-      const double *r = brdf + ((size_t)brdf_index[w] * S + s) * blk;
-      for (size_t k = 0; k < blk; ++k) o[k] = o[k] * r[k];
-    }
+    fft_pair(i, j, f, B, centers, air, wall_ids, S, D, brdf, brdf_index,
+             sources, out + ((size_t)i * P + j) * blk);
+  }
+}
+
+/* _form_factors_with_directivity_dim (RadiosityFast.py:1234-1272) for a list
+ * of directed visible pairs (i -> j, int64 [n,2]); ff_upper[q] = ff[min,
+ * max] of pair q (the i < j form factor).  out: [n, D, B]. */
+void orc_fft_pairs(int64_t n, const int64_t *pairs, const double *ff_upper,
+                   int64_t B, const double *centers, const double *areas,
+                   const double *air, const int64_t *wall_ids, int64_t S,
+                   int64_t D, const double *brdf, const int64_t *brdf_index,
+                   const double *sources, double *out) {
+  size_t blk = (size_t)D * B;
+  for (int64_t q = 0; q < n; ++q) {
+    int64_t i = pairs[2 * q], j = pairs[2 * q + 1];
+    double f = (i < j) ? ff_upper[q] : (ff_upper[q] * areas[j]) / areas[i];
+    fft_pair(i, j, f, B, centers, air, wall_ids, S, D, brdf, brdf_index,
+             sources, out + (size_t)q * blk);
   }
 }
 
@@ -745,22 +801,32 @@ int orc_init_energy(int64_t T, int64_t P, int64_t D, int64_t B,
   return 0;
 }
 
-/* accumulate one directed pair i -> j into cur (RadiosityFast.py:1133-1143) */
-static inline void pair_update(int64_t T, int64_t D, int64_t B,
-                               const double *prev, double *cur, int64_t i,
-                               int64_t j, int64_t dir, int64_t n,
-                               const double *f) {
+/* accumulate one directed pair into a destination (RadiosityFast.py:
+ * 1133-1143): dst[d, b, t] += f[d, b] * src[b, t - n], src the source's
+ * [B, T] rows of its outgoing direction, dst the destination's [D, B, T]. */
+static inline void row_update(int64_t T, int64_t D, int64_t B,
+                              const double *src_rows, double *dst_rows,
+                              int64_t n, const double *f) {
   for (int64_t d = 0; d < D; ++d)
     for (int64_t b = 0; b < B; ++b) {
       double fv = f[d * B + b];
-      const double *src = prev + (((size_t)i * D + dir) * B + b) * T;
-      double *dst = cur + (((size_t)j * D + d) * B + b) * T;
+      const double *src = src_rows + (size_t)b * T;
+      double *dst = dst_rows + ((size_t)d * B + b) * T;
       if (n > 0) {
         for (int64_t t = n; t < T; ++t) dst[t] = dst[t] + fv * src[t - n];
       } else {
         for (int64_t t = 0; t < T; ++t) dst[t] = dst[t] + fv * src[t];
       }
     }
+}
+
+/* accumulate one directed pair i -> j into cur (RadiosityFast.py:1133-1143) */
+static inline void pair_update(int64_t T, int64_t D, int64_t B,
+                               const double *prev, double *cur, int64_t i,
+                               int64_t j, int64_t dir, int64_t n,
+                               const double *f) {
+  row_update(T, D, B, prev + (((size_t)i * D + dir) * B) * T,
+             cur + ((size_t)j * D * B) * T, n, f);
 }
 
 /* RadiosityFast.py:1073-1145 _energy_exchange */
@@ -830,6 +896,89 @@ int orc_exchange(int64_t T, int64_t P, int64_t D, int64_t B,
   return 0;
 }
 
+/* One order of _energy_exchange (RadiosityFast.py:1121-1143) on the dense
+ * arrays: cur = the order after prev (overwritten), per destination in
+ * ascending source order (the threaded order of orc_exchange). */
+void orc_exchange_step(int64_t T, int64_t P, int64_t D, int64_t B,
+                       const double *prev, double *cur, const double *dist,
+                       const double *fft, const int64_t *p2o, double c,
+                       double dt, const uint8_t *vis, int nthreads) {
+  size_t blk = (size_t)D * B;
+#ifdef _OPENMP
+  if (nthreads < 1) nthreads = 1;
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads)
+#endif
+  for (int64_t j = 0; j < P; ++j) {
+    memset(cur + (size_t)j * blk * T, 0, sizeof(double) * blk * T);
+    for (int64_t i = 0; i < P; ++i) {
+      if (i == j || !(i < j ? vis[i * P + j] : vis[j * P + i])) continue;
+      int64_t n = orc_delay_floor(dist[i * P + j], c, dt);
+      if (n >= T) continue;
+      pair_update(T, D, B, prev, cur, i, j, p2o[i * P + j], n,
+                  fft + ((size_t)i * P + j) * blk);
+    }
+  }
+}
+
+/* Adjoint of one order of _energy_exchange (the transpose of
+ * RadiosityFast.py:1133-1143; no counterpart in the reference):
+ * mu_out = lam + A^T mu_next, i.e. for every directed visible pair i -> j
+ * with delay n < T and outgoing direction d = p2o[i,j]:
+ *   mu_out[i,d,b,t] += sum_e fft[i,j,e,b] mu_next[j,e,b,t+n]   (t < T-n)
+ *   dfft[i,j,e,b]   += sum_t mu_next[j,e,b,t+n] ek[i,d,b,t]
+ * (dfft accumulates over calls).  Parallel over sources i, so every write
+ * has one owner. */
+void orc_exchange_adjoint_step(int64_t T, int64_t P, int64_t D, int64_t B,
+                               const double *lam, const double *mu_next,
+                               const double *ek, const double *dist,
+                               const double *fft, const int64_t *p2o,
+                               double c, double dt, const uint8_t *vis,
+                               double *mu_out, double *dfft, int nthreads) {
+  size_t blk = (size_t)D * B;
+  memcpy(mu_out, lam, sizeof(double) * (size_t)P * blk * T);
+#ifdef _OPENMP
+  if (nthreads < 1) nthreads = 1;
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads)
+#endif
+  for (int64_t i = 0; i < P; ++i) {
+    for (int64_t j = 0; j < P; ++j) {
+      if (i == j || !(i < j ? vis[i * P + j] : vis[j * P + i])) continue;
+      int64_t n = orc_delay_floor(dist[i * P + j], c, dt);
+      if (n >= T) continue;
+      int64_t d = p2o[i * P + j];
+      const double *f = fft + ((size_t)i * P + j) * blk;
+      double *g = dfft + ((size_t)i * P + j) * blk;
+      for (int64_t b = 0; b < B; ++b) {
+        double *mo = mu_out + (((size_t)i * D + d) * B + b) * T;
+        const double *e = ek + (((size_t)i * D + d) * B + b) * T;
+        for (int64_t q = 0; q < D; ++q) {
+          const double *mn = mu_next + (((size_t)j * D + q) * B + b) * T + n;
+          double fv = f[q * B + b], acc = 0.0;
+          for (int64_t t = 0; t < T - n; ++t) {
+            mo[t] = mo[t] + fv * mn[t];
+            acc = acc + mn[t] * e[t];
+          }
+          g[q * B + b] = g[q * B + b] + acc;
+        }
+      }
+    }
+  }
+}
+
+/* get_scattering_data_source's index (RadiosityFast.py:1359-1390) for a list
+ * of directed pairs (i -> j): the incoming direction of wall_i nearest to
+ * unit(c_i - c_j), as orc_fft picks it.  out[n]. */
+void orc_source_index_pairs(int64_t n, const int64_t *pairs,
+                            const double *centers, int64_t S,
+                            const double *sources, const int64_t *wall_ids,
+                            int64_t *out) {
+  for (int64_t q = 0; q < n; ++q) {
+    int64_t i = pairs[2 * q], j = pairs[2 * q + 1];
+    out[q] = orc_source_index(centers + 3 * i, centers + 3 * j, S, sources,
+                              wall_ids[i]);
+  }
+}
+
 void orc_exchange_order(int64_t T, int64_t P, int64_t D, int64_t B,
                         const double *prev, double *cur,
                         const int64_t *row_ptr, const int32_t *row_src,
@@ -847,6 +996,30 @@ void orc_exchange_order(int64_t T, int64_t P, int64_t D, int64_t B,
     for (int64_t e = row_ptr[j]; e < row_ptr[j + 1]; ++e)
       pair_update(T, D, B, prev, cur, row_src[e], j, row_dir[e], row_lag[e],
                   fft + (size_t)e * blk);
+  }
+}
+
+/* One order of _energy_exchange (RadiosityFast.py:1121-1143) for n_dest
+ * destinations whose source rows are given compactly: entry e of
+ * destination q (row_ptr[q] <= e < row_ptr[q+1], in accumulation order)
+ * reads prev_rows[row_src[e], :, :] ([n_rows, B, T], the source patch's
+ * histogram of the outgoing direction towards q) delayed by row_lag[e],
+ * with coefficients coef[e, D, B].  cur: [n_dest, D, B, T], overwritten. */
+void orc_exchange_rows(int64_t T, int64_t D, int64_t B, int64_t n_dest,
+                       const int64_t *row_ptr, const int64_t *row_src,
+                       const int64_t *row_lag, const double *coef,
+                       const double *prev_rows, double *cur, int nthreads) {
+  size_t blk = (size_t)D * B;
+#ifdef _OPENMP
+  if (nthreads < 1) nthreads = 1;
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads)
+#endif
+  for (int64_t q = 0; q < n_dest; ++q) {
+    double *dst = cur + (size_t)q * blk * T;
+    memset(dst, 0, sizeof(double) * blk * T);
+    for (int64_t e = row_ptr[q]; e < row_ptr[q + 1]; ++e)
+      row_update(T, D, B, prev_rows + (size_t)row_src[e] * B * T, dst,
+                 row_lag[e], coef + (size_t)e * blk);
   }
 }
 

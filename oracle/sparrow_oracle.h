@@ -34,6 +34,14 @@ void orc_p2p_visibility(int64_t P, const double *centers, int64_t NS,
                         const double *surf_normals, const double *surf_pts,
                         int npts, uint8_t *out, int nthreads);
 
+/* geometry.py:750-797 for a list of pairs (i < j each, int64 [n,2]):
+ * out[q] = visibility of pair q (sampled rows of large scenes). */
+void orc_p2p_visibility_pairs(const double *centers, int64_t NS,
+                              const double *surf_normals,
+                              const double *surf_pts, int npts, int64_t n,
+                              const int64_t *pairs, uint8_t *out,
+                              int nthreads);
+
 /* geometry.py:799-839 _check_point2patch_visibility: out (P,) */
 void orc_point_visibility(const double *point, int64_t P,
                           const double *centers, int64_t NS,
@@ -95,6 +103,15 @@ void orc_fft(int64_t P, const uint8_t *vis, const double *ff, int64_t B,
              const double *brdf, const int64_t *brdf_index,
              const double *sources, double *out, int nthreads);
 
+/* RadiosityFast.py:1234-1272 for a list of directed visible pairs
+ * (i -> j, int64 [n,2]); ff_upper[q] = ff[min(i,j), max(i,j)].
+ * out (n,D,B). */
+void orc_fft_pairs(int64_t n, const int64_t *pairs, const double *ff_upper,
+                   int64_t B, const double *centers, const double *areas,
+                   const double *air, const int64_t *wall_ids, int64_t S,
+                   int64_t D, const double *brdf, const int64_t *brdf_index,
+                   const double *sources, double *out);
+
 /* RadiosityFast.py:988-1034 _add_directional: out (P,D,B) */
 void orc_add_directional(int64_t P, int64_t B, const double *energy0,
                          const double *source, const double *centers,
@@ -120,6 +137,29 @@ int orc_exchange(int64_t T, int64_t P, int64_t D, int64_t B,
                  int64_t K, int64_t V, const int32_t *pairs, double *out,
                  double *work, int nthreads);
 
+/* One order of the exchange on dense arrays (vis (P,P) upper triangle,
+ * dist, fft, p2o as orc_exchange): cur (P,D,B,T) overwritten. */
+void orc_exchange_step(int64_t T, int64_t P, int64_t D, int64_t B,
+                       const double *prev, double *cur, const double *dist,
+                       const double *fft, const int64_t *p2o, double c,
+                       double dt, const uint8_t *vis, int nthreads);
+
+/* Adjoint of one order (transpose of RadiosityFast.py:1133-1143):
+ * mu_out = lam + A^T mu_next; dfft (P,P,D,B) += per-pair coefficient
+ * gradients against the forward order ek.  No reference counterpart. */
+void orc_exchange_adjoint_step(int64_t T, int64_t P, int64_t D, int64_t B,
+                               const double *lam, const double *mu_next,
+                               const double *ek, const double *dist,
+                               const double *fft, const int64_t *p2o,
+                               double c, double dt, const uint8_t *vis,
+                               double *mu_out, double *dfft, int nthreads);
+
+/* RadiosityFast.py:1359-1390 source index for directed pairs (n,2). */
+void orc_source_index_pairs(int64_t n, const int64_t *pairs,
+                            const double *centers, int64_t S,
+                            const double *sources, const int64_t *wall_ids,
+                            int64_t *out);
+
 /* One reflection order of the exchange restricted to destinations
  * [j_begin, j_end) of the per-destination lists (CPU timing sample).
  * cur is overwritten on those destinations. */
@@ -129,6 +169,14 @@ void orc_exchange_order(int64_t T, int64_t P, int64_t D, int64_t B,
                         const int32_t *row_lag, const int32_t *row_dir,
                         const double *fft, int64_t j_begin, int64_t j_end,
                         int nthreads);
+
+/* One order (RadiosityFast.py:1121-1143) for n_dest destinations with
+ * compact source rows: prev_rows (n_rows,B,T), per entry row_src (index
+ * into prev_rows), row_lag, coef (e,D,B); cur (n_dest,D,B,T). */
+void orc_exchange_rows(int64_t T, int64_t D, int64_t B, int64_t n_dest,
+                       const int64_t *row_ptr, const int64_t *row_src,
+                       const int64_t *row_lag, const double *coef,
+                       const double *prev_rows, double *cur, int nthreads);
 
 /* RadiosityFast.py:1148-1185 _collect_receiver_energy:
  * E (P,B,T), dist (P), air (B) -> out (P,B,T) */
