@@ -25,7 +25,10 @@ import subprocess
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+# ORACLE_LIB selects another build of the same sources (the sanitizer build
+# of ``make sanitize``, loaded by tests/test_sanitizers.py)
+_LIB_PATH = os.environ.get("ORACLE_LIB") or os.path.join(_HERE,
+                                                         "liboracle.so")
 _lib = None
 
 _f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
