@@ -443,6 +443,28 @@ int spr_energy_exchange(int64_t T, int64_t P, int64_t D, int64_t B,
                         void *workspace, int64_t workspace_bytes,
                         void *stream);
 
+/* Scratch built into the library for spr_energy_exchange_noalloc, per GPU
+ * (64 MiB: enough for the 148-patch shoebox with 16 directions, one band and
+ * 1,000 bins, which needs about 31 MB). */
+#define SPR_NOALLOC_SCRATCH_BYTES (64ll << 20)
+
+/* No counterpart in the reference.  spr_energy_exchange for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc): the same arguments minus
+ * the workspace, the same kernels and results.  Its workspace is a
+ * file-scope device array of SPR_NOALLOC_SCRATCH_BYTES, one per GPU, held
+ * until the process ends; the call allocates and frees nothing.  Returns
+ * SPR_ERR_WORKSPACE when spr_energy_exchange_workspace(P, D, B, T, V)
+ * exceeds SPR_NOALLOC_SCRATCH_BYTES.  Every call on one GPU uses that GPU's
+ * array, so calls that overlap on one GPU (other threads or streams) race;
+ * serialise them.  Synchronous, like spr_energy_exchange. */
+int spr_energy_exchange_noalloc(int64_t T, int64_t P, int64_t D, int64_t B,
+                                const double *e0dir, const double *d0,
+                                const double *dist, const double *fft,
+                                const int64_t *p2o, double c, double dt,
+                                int64_t K, int64_t V, const int32_t *pairs,
+                                float *etc, void *stream);
+
 /* ---------------------------------------------------------------------
  * Receiver collection
  * ------------------------------------------------------------------- */

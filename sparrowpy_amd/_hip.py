@@ -134,6 +134,9 @@ SYMBOLS = {
     "spr_energy_exchange": (_int, [_i64, _i64, _i64, _i64, _vp, _vp, _vp,
                                    _vp, _vp, _dbl, _dbl, _i64, _i64, _vp,
                                    _vp, _vp, _i64, _vp]),
+    "spr_energy_exchange_noalloc": (_int, [_i64, _i64, _i64, _i64, _vp, _vp,
+                                           _vp, _vp, _vp, _dbl, _dbl, _i64,
+                                           _i64, _vp, _vp, _vp]),
     "spr_receiver_energy": (_int, [_i64, _vp, _i64, _vp, _vp, _vp, _vp]),
     "spr_collect": (_int, [_i64, _i64, _i64, _i64, _i64, _vp, _vp, _vp, _vp,
                            _vp, _vp, _dbl, _dbl, _int, _vp, _vp]),

@@ -16,6 +16,8 @@
 // contiguous 256-byte wavefront access) and accumulates all DP directions
 // in float64 registers.  The per-pair coefficient row coef[e, b, :] is
 // uniform across the workgroup and is read through the scalar unit.
+#include <mutex>
+
 #include "spr_common.hpp"
 #include "spr_fft.hpp"
 #include "spr_util.hpp"
@@ -1740,30 +1742,16 @@ extern "C" int64_t spr_energy_exchange_workspace(int64_t P, int64_t D,
   return (int64_t)dense_layout(P, D, B, T, V).total;
 }
 
-extern "C" int spr_energy_exchange(int64_t T, int64_t P, int64_t D, int64_t B,
-                                   const double *e0dir, const double *d0,
-                                   const double *dist, const double *fft,
-                                   const int64_t *p2o, double c, double dt,
-                                   int64_t K, int64_t V,
-                                   const int32_t *pairs, float *etc,
-                                   void *workspace, int64_t workspace_bytes,
-                                   void *stream) {
-  SPR_REQUIRE(T > 0 && P > 0 && D > 0 && B > 0 && V >= 0,
-              "spr_energy_exchange: bad sizes");
-  SPR_REQUIRE(spr_padded_directions(D) > 0,
-              "spr_energy_exchange: D=%lld unsupported (max 64)",
-              (long long)D);
-  SPR_REQUIRE(e0dir && d0 && etc && workspace,
-              "spr_energy_exchange: null pointer");
-  SPR_REQUIRE(K < 1 || (dist && fft && p2o && (pairs || V == 0)),
-              "spr_energy_exchange: null pointer");
-  DenseLayout L = dense_layout(P, D, B, T, V);
-  if ((int64_t)L.total > workspace_bytes)
-    return fail(SPR_ERR_WORKSPACE,
-                "spr_energy_exchange: workspace of %lld bytes, need %lld",
-                (long long)workspace_bytes, (long long)L.total);
+namespace {
+// The body of spr_energy_exchange on a workspace `ws` of the dense layout
+// (sizes already checked by the caller).
+int energy_exchange_on(int64_t T, int64_t P, int64_t D, int64_t B,
+                       const double *e0dir, const double *d0,
+                       const double *dist, const double *fft,
+                       const int64_t *p2o, double c, double dt, int64_t K,
+                       int64_t V, const int32_t *pairs, float *etc, char *ws,
+                       const DenseLayout &L, void *stream) {
   hipStream_t s = as_stream(stream);
-  char *ws = static_cast<char *>(workspace);
   auto *vis = reinterpret_cast<uint8_t *>(ws + L.vis);
   auto *row_ptr = reinterpret_cast<int64_t *>(ws + L.row_ptr);
   auto *src = reinterpret_cast<int32_t *>(ws + L.src);
@@ -1808,4 +1796,93 @@ extern "C" int spr_energy_exchange(int64_t T, int64_t P, int64_t D, int64_t B,
                 "spr_energy_exchange: a source delay is beyond the "
                 "histogram length");
   return SPR_OK;
+}
+
+int check_exchange_args(const char *name, int64_t T, int64_t P, int64_t D,
+                        int64_t B, const double *e0dir, const double *d0,
+                        const double *dist, const double *fft,
+                        const int64_t *p2o, int64_t K, int64_t V,
+                        const int32_t *pairs, const float *etc) {
+  SPR_REQUIRE(T > 0 && P > 0 && D > 0 && B > 0 && V >= 0,
+              "%s: bad sizes", name);
+  SPR_REQUIRE(spr_padded_directions(D) > 0,
+              "%s: D=%lld unsupported (max 64)", name, (long long)D);
+  SPR_REQUIRE(e0dir && d0 && etc, "%s: null pointer", name);
+  SPR_REQUIRE(K < 1 || (dist && fft && p2o && (pairs || V == 0)),
+              "%s: null pointer", name);
+  return SPR_OK;
+}
+
+// Scratch of spr_energy_exchange_noalloc: one copy per GPU, held until the
+// process ends.  Host code never uses `scratch` itself (on the host the name
+// stands for a host-side shadow, not device memory): the device address of
+// the current GPU's copy is looked up once per device with
+// hipGetSymbolAddress and kept in noalloc_ptr.
+static __device__ __align__(256) unsigned char
+    scratch[SPR_NOALLOC_SCRATCH_BYTES];
+
+constexpr int kMaxDevices = 64;
+std::mutex noalloc_mutex;
+void *noalloc_ptr[kMaxDevices] = {};
+
+int noalloc_scratch(void **out) {
+  int dev = -1;
+  SPR_HIP(hipGetDevice(&dev));
+  SPR_REQUIRE(dev >= 0 && dev < kMaxDevices,
+              "spr_energy_exchange_noalloc: device %d out of range", dev);
+  std::lock_guard<std::mutex> lock(noalloc_mutex);
+  if (!noalloc_ptr[dev]) {
+    void *p = nullptr;
+    SPR_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    SPR_REQUIRE(p != nullptr,
+                "spr_energy_exchange_noalloc: no scratch on device %d", dev);
+    noalloc_ptr[dev] = p;
+  }
+  *out = noalloc_ptr[dev];
+  return SPR_OK;
+}
+}  // namespace
+
+extern "C" int spr_energy_exchange(int64_t T, int64_t P, int64_t D, int64_t B,
+                                   const double *e0dir, const double *d0,
+                                   const double *dist, const double *fft,
+                                   const int64_t *p2o, double c, double dt,
+                                   int64_t K, int64_t V,
+                                   const int32_t *pairs, float *etc,
+                                   void *workspace, int64_t workspace_bytes,
+                                   void *stream) {
+  int rc = check_exchange_args("spr_energy_exchange", T, P, D, B, e0dir, d0,
+                               dist, fft, p2o, K, V, pairs, etc);
+  if (rc) return rc;
+  SPR_REQUIRE(workspace, "spr_energy_exchange: null pointer");
+  DenseLayout L = dense_layout(P, D, B, T, V);
+  if ((int64_t)L.total > workspace_bytes)
+    return fail(SPR_ERR_WORKSPACE,
+                "spr_energy_exchange: workspace of %lld bytes, need %lld",
+                (long long)workspace_bytes, (long long)L.total);
+  return energy_exchange_on(T, P, D, B, e0dir, d0, dist, fft, p2o, c, dt, K,
+                            V, pairs, etc, static_cast<char *>(workspace), L,
+                            stream);
+}
+
+extern "C" int spr_energy_exchange_noalloc(
+    int64_t T, int64_t P, int64_t D, int64_t B, const double *e0dir,
+    const double *d0, const double *dist, const double *fft,
+    const int64_t *p2o, double c, double dt, int64_t K, int64_t V,
+    const int32_t *pairs, float *etc, void *stream) {
+  clear_error();
+  int rc = check_exchange_args("spr_energy_exchange_noalloc", T, P, D, B,
+                               e0dir, d0, dist, fft, p2o, K, V, pairs, etc);
+  if (rc) return rc;
+  DenseLayout L = dense_layout(P, D, B, T, V);
+  if ((int64_t)L.total > (int64_t)SPR_NOALLOC_SCRATCH_BYTES)
+    return fail(SPR_ERR_WORKSPACE,
+                "spr_energy_exchange_noalloc: the scene needs %lld bytes of "
+                "scratch, more than the %lld bytes built in; use "
+                "spr_energy_exchange with a workspace",
+                (long long)L.total, (long long)SPR_NOALLOC_SCRATCH_BYTES);
+  void *ws = nullptr;
+  if ((rc = noalloc_scratch(&ws))) return rc;
+  return energy_exchange_on(T, P, D, B, e0dir, d0, dist, fft, p2o, c, dt, K,
+                            V, pairs, etc, static_cast<char *>(ws), L, stream);
 }

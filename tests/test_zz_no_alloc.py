@@ -1,0 +1,124 @@
+"""spr_energy_exchange_noalloc: the exchange without a caller workspace.
+
+The entry point runs the kernels of spr_energy_exchange on a scratch array
+built into the library (one per GPU) instead of a workspace argument.  Its
+histograms must equal spr_energy_exchange's bit for bit (same kernels, same
+order of operations), on two calls in a row where the second scene is
+smaller (the scratch keeps the first call's contents), and stay within the
+exchange tolerance of the FP64 oracle.  Sizes beyond the built-in scratch
+are rejected before any device work.
+"""
+import ctypes
+
+import numpy as np
+import pytest
+
+import oracle as orc
+from conftest import make_oracle, rel_err_within_db
+from sparrowpy_amd import brdf as B
+from sparrowpy_amd import samplings, testing
+from sparrowpy_amd.coordinates import FrequencyData
+
+pytestmark = pytest.mark.gpu
+
+EXCHANGE_RTOL = 1e-5  # per-bin, bins within 60 dB of the histogram peak
+
+
+@pytest.fixture(scope="module")
+def hip():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from sparrowpy_amd import _hip
+    return _hip
+
+
+def _scene(dims, patch, source, n_bands, seed):
+    rng = np.random.default_rng(seed)
+    r = make_oracle(testing.shoebox_room_stub(*dims), patch)
+    c = samplings.hemisphere(samplings.sph_gaussian(sh_order=3))
+    freqs = np.array([250., 500., 1000., 2000.][:n_bands])
+    for w in range(r.n_walls):
+        br = B.create_from_scattering(
+            c, c, FrequencyData(rng.uniform(.1, .9, (1, n_bands)), freqs),
+            FrequencyData(rng.uniform(.05, .3, (1, n_bands)), freqs))
+        r.set_wall_brdf(np.atleast_1d(w), br.freq, c.cartesian, c.cartesian,
+                        freqs)
+    r.set_air_attenuation(rng.uniform(1e-4, 5e-3, n_bands), freqs)
+    r.bake_geometry()
+    r.init_source_energy(np.asarray(source, float))
+    return r
+
+
+def _run(hip, name, r, T, c, dt, K):
+    """Call `name` through the C ABI; returns the float32 histogram."""
+    import torch
+    dev = torch.device("cuda", 0)
+    f64 = dict(dtype=torch.float64, device=dev)
+    e0d = torch.tensor(r.energy_init_source, **f64)
+    d0 = torch.tensor(r.distance_patches_to_source, **f64)
+    dist = torch.tensor(r.pair_distances(), **f64)
+    fft = torch.tensor(r.fft, **f64)
+    p2o = torch.tensor(r.p2o, dtype=torch.int64, device=dev)
+    pairs = torch.tensor(np.asarray(r.visible_patches, np.int32), device=dev)
+    P, D, Bn = e0d.shape
+    V = pairs.shape[0]
+    etc = torch.full((P, D, Bn, T), float("nan"), dtype=torch.float32,
+                     device=dev)
+    args = [T, P, D, Bn, hip.ptr(e0d), hip.ptr(d0), hip.ptr(dist),
+            hip.ptr(fft), hip.ptr(p2o), c, dt, K, V, hip.ptr(pairs),
+            hip.ptr(etc)]
+    if name == "spr_energy_exchange":
+        nws = int(hip.lib().spr_energy_exchange_workspace(P, D, Bn, T, V))
+        ws = torch.empty(nws, dtype=torch.uint8, device=dev)
+        args += [hip.ptr(ws), nws]
+    print(f"[no-alloc] {name}: P={P} D={D} B={Bn} T={T} K={K} V={V}",
+          flush=True)
+    hip.call(name, *args, hip.stream_ptr())
+    torch.cuda.synchronize()
+    return etc.cpu().numpy()
+
+
+CASES = [
+    # the larger scene first, then a smaller one on the same scratch
+    dict(dims=(3, 2.5, 2), patch=.5, source=[1.1, .8, .6], n_bands=2,
+         seed=0, T=400, dt=1 / 2000, K=8),
+    dict(dims=(2, 2, 1), patch=.5, source=[.7, 1.2, .4], n_bands=1,
+         seed=1, T=150, dt=1 / 1000, K=5),
+]
+
+
+def test_noalloc_matches_workspace_entry_point(hip):
+    c = 343.
+    for case in CASES:
+        r = _scene(case["dims"], case["patch"], case["source"],
+                   case["n_bands"], case["seed"])
+        T, dt, K = case["T"], case["dt"], case["K"]
+        ws = _run(hip, "spr_energy_exchange", r, T, c, dt, K)
+        na = _run(hip, "spr_energy_exchange_noalloc", r, T, c, dt, K)
+        assert np.isfinite(na).all()
+        np.testing.assert_array_equal(na, ws)
+        ref = orc.exchange(T, r.energy_init_source,
+                           r.distance_patches_to_source, r.pair_distances(),
+                           r.fft, r.p2o, c, dt, K, r.visible_patches,
+                           nthreads=8)
+        err = rel_err_within_db(na, ref)
+        print(f"[no-alloc] P={r.n_patches} T={T}: equal to the workspace "
+              f"entry point; max rel err vs oracle {err:.3e}", flush=True)
+        assert err <= EXCHANGE_RTOL
+
+
+def test_noalloc_rejects_oversized_scene(hip):
+    """A scene whose workspace exceeds the built-in scratch is refused
+    with SPR_ERR_WORKSPACE before any device work (null data pointers are
+    never dereferenced: the size check comes first)."""
+    lib = hip.lib()
+    P, D, Bn, T, V = 5000, 32, 8, 6000, 1000
+    need = int(lib.spr_energy_exchange_workspace(P, D, Bn, T, V))
+    assert need > (64 << 20)
+    fake = ctypes.c_void_p(256)  # non-null, never dereferenced
+    rc = lib.spr_energy_exchange_noalloc(T, P, D, Bn, fake, fake, fake, fake,
+                                         fake, 343., 1 / 4000, 3, V, fake,
+                                         fake, None)
+    assert rc == -5
+    assert b"scratch" in lib.spr_last_error()
