@@ -55,6 +55,7 @@ class ExchangeAdjoint:
         self.T = int(duration / time_resolution)
         self.K = int(max_order)
         self.every = max(1, int(checkpoint_every))
+        self.last = None   # ForwardRecord of the last forward()
         plan = ops.diffuse_specular_plan(
             b["vis"], g["centers"], g["areas"], b["ff"], b["p2o"], b["air"],
             g["wall_ids"], b["sources"], b["brdf"], b["brdf_index"], b["D"],
@@ -143,17 +144,24 @@ class ExchangeAdjoint:
              None, None, stream_ptr())
 
     def forward(self, A, C):
-        """Receiver histograms (float32 [R, B, T]) for tables (A, C);
-        keeps the order checkpoints for :meth:`backward`."""
+        """Receiver histograms (float32 [R, B, T]) for tables (A, C).
+
+        The call's order checkpoints and tables are kept as ``self.last``
+        (a :class:`ForwardRecord`) for :meth:`backward`; a later forward
+        replaces it, so the autograd entry takes each call's record into
+        its own context instead."""
+        A = ops.as_dev(A, torch.float64).detach().clone()
+        C = ops.as_dev(C, torch.float64).detach().clone()
         self._set_tables(A, C)
         st = ops.ExchangeState(self.P, self.D, self.B, self.T, self.dev)
         st.init(self.e0dir, self.d0, self.c, self.dt)
-        self.checkpoints = {0: st.prev.clone()}
+        checkpoints = {0: st.prev.clone()}
         for k in range(self.K):
             st.step(self.plan)
             if (k + 1) % self.every == 0 and k + 1 < self.K:
-                self.checkpoints[k + 1] = st.prev.clone()
+                checkpoints[k + 1] = st.prev.clone()
         st.check()
+        self.last = ForwardRecord(A, C, checkpoints)
         total = st.finalize(out=st.hi)
         self._gsum = st.gsum if st.gsum is not None else torch.empty(
             (max(self.plan.n_groups, 1), self.B, self.T), dtype=torch.float32,
@@ -164,9 +172,19 @@ class ExchangeAdjoint:
         return H
 
     # -- backward ---------------------------------------------------------------
-    def backward(self, grad_hist):
+    def backward(self, grad_hist, record=None):
         """(dL/dA, dL/dC) (float64 [NK, B]) for dL/dH = ``grad_hist``
-        ([R, B, T]) at the tables of the last :meth:`forward`."""
+        ([R, B, T]) at the tables of ``record`` (default: the last
+        :meth:`forward`).  The record's tables are set again first, so
+        the recomputed orders and the transposed weights are those of the
+        forward the record comes from."""
+        record = self.last if record is None else record
+        if record is None or not record.checkpoints:
+            raise RuntimeError("ExchangeAdjoint.backward: no forward record "
+                               "(forward() was not called, or its record "
+                               "was released)")
+        self._set_tables(record.A, record.C)
+        checkpoints = record.checkpoints
         P, D, B, T, Vd = self.P, self.D, self.B, self.T, self.plan.Vd
         dev, p, adj = self.dev, self.plan, self.plan.adjoint
         gH = ops.as_dev(grad_hist, torch.float32).contiguous()
@@ -184,10 +202,13 @@ class ExchangeAdjoint:
         musum = torch.empty((P, B, T), dtype=torch.float32, device=dev)
         mu = lam.clone()                     # mu_K
         mu_out = torch.empty_like(lam)
-        starts = sorted(self.checkpoints)
+        starts = sorted(checkpoints)
+        if starts != list(range(0, max(self.K, 1), self.every)):
+            raise RuntimeError("ExchangeAdjoint.backward: the record's "
+                               "checkpoints do not cover the orders")
         for s in reversed(starts):
             end = min(s + self.every, self.K)
-            states = [self.checkpoints[s]]   # E_s .. E_{end-1}
+            states = [checkpoints[s]]   # E_s .. E_{end-1}
             for k in range(s, end - 1):
                 nxt = torch.empty_like(lam)
                 self._order(states[-1], nxt, self._gsum)
@@ -212,20 +233,31 @@ class ExchangeAdjoint:
         return gA, gC
 
     def release(self):
-        """Drop the checkpoints of the last forward."""
-        self.checkpoints = {}
+        """Drop the record of the last forward."""
+        self.last = None
+
+
+class ForwardRecord:
+    """What the adjoint of one forward call needs: its tables (A, C) and
+    the order histograms kept every ``checkpoint_every`` orders."""
+
+    def __init__(self, A, C, checkpoints):
+        self.A, self.C, self.checkpoints = A, C, checkpoints
 
 
 class _Histograms(torch.autograd.Function):
     @staticmethod
     def forward(ctx, A, C, adj):
         ctx.adj = adj
-        return adj.forward(A.detach(), C.detach()).to(torch.float64)
+        H = adj.forward(A.detach(), C.detach()).to(torch.float64)
+        # each call keeps its own record (freed with the graph), so several
+        # calls in one graph, and backward with retain_graph, stay correct
+        ctx.record, adj.last = adj.last, None
+        return H
 
     @staticmethod
     def backward(ctx, grad):
-        gA, gC = ctx.adj.backward(grad)
-        ctx.adj.release()
+        gA, gC = ctx.adj.backward(grad, ctx.record)
         return gA, gC, None
 
 

@@ -100,6 +100,59 @@ def test_gradient_matches_fp64_reference(torch, every):
     assert err <= 1e-4
 
 
+def test_two_calls_in_one_graph(torch):
+    """Two receiver_histograms calls with different tables in one loss,
+    and a second backward through the same graph (retain_graph): each call
+    keeps its own checkpoints and tables, so the gradient is the sum of the
+    FP64 reference gradients at both points (twice that after the second
+    backward)."""
+    from sparrowpy_amd import adjoint
+    walls, dirs, ta, ts, alpha, scat, src, rcv, (T, c, dt, K) = _setup()
+    target = run(oracle_scene(walls, dirs, ta, ts), src, rcv, T, c, dt, K)
+    sv, m = spec_info(dirs)
+    delta = np.full_like(alpha, .03)
+    ga_ref = np.zeros_like(alpha)
+    gs_ref = np.zeros_like(scat)
+    L_ref = 0.0
+    for a_np in (alpha, alpha + delta):
+        ro = oracle_scene(walls, dirs, a_np, scat)
+        L, gH = loss_and_grad(run(ro, src, rcv, T, c, dt, K), target)
+        ga, gs = oadj.scattering_gradient(oadj.gradient(ro, rcv, gH, K),
+                                          a_np, scat, sv, m)
+        ga_ref += ga
+        gs_ref += gs
+        L_ref += L
+    _, adj = _adjoint(torch, walls, dirs, alpha, scat, src, rcv, T, c, dt, K,
+                      2)
+    kw = dict(dtype=torch.float64, device="cuda")
+    a = torch.tensor(alpha, requires_grad=True, **kw)
+    s = torch.tensor(scat, requires_grad=True, **kw)
+    d = torch.tensor(delta, **kw)
+    H1 = adjoint.receiver_histograms(adj, *adjoint.scattering_tables(a, s,
+                                                                      sv))
+    H2 = adjoint.receiver_histograms(adj, *adjoint.scattering_tables(a + d,
+                                                                      s, sv))
+    loss = adjoint.log_energy_loss(H1, target) + \
+        adjoint.log_energy_loss(H2, target)
+    loss.backward(retain_graph=True)
+    ga, gs = a.grad.cpu().numpy().copy(), s.grad.cpu().numpy().copy()
+    gmax = max(np.abs(ga_ref).max(), np.abs(gs_ref).max())
+    err = max(np.abs(ga - ga_ref).max(), np.abs(gs - gs_ref).max()) / gmax
+    print(f"[adjoint two calls] loss gpu {float(loss):.6e} fp64 "
+          f"{L_ref:.6e}, max grad err / max grad {err:.2e}")
+    assert abs(float(loss) - L_ref) <= 1e-5 * L_ref
+    assert err <= 1e-4
+    loss.backward()
+    ga2, gs2 = a.grad.cpu().numpy(), s.grad.cpu().numpy()
+    err2 = max(np.abs(ga2 - 2 * ga_ref).max(),
+               np.abs(gs2 - 2 * gs_ref).max()) / (2 * gmax)
+    assert err2 <= 1e-4
+    # without a record, backward refuses instead of dropping orders
+    adj.release()
+    with pytest.raises(RuntimeError, match="no forward record"):
+        adj.backward(torch.zeros((len(rcv), 2, T), **kw))
+
+
 def test_gradient_matches_finite_differences(torch):
     """GPU gradient against central differences (h = 1e-4) of the FP64
     oracle loss on 50 random parameters."""
