@@ -1,0 +1,377 @@
+"""Oracle parity at the benchmark workloads themselves.
+
+* Street canyon (5,122 patches, 32 directions, 8 bands, 6,000 bins; the
+  throughput workload of bench.py) at full size: for 32 seeded destination
+  patches, visibility against every other patch, the direction map, the
+  exchange plan's source lists and integer delays are bit-exact against the
+  FP64 oracle; two orders run on the plan bench.py uses (grouped, on
+  destination tiles), and the second order's histograms of those
+  destinations (all directions, all 8 bands, all 6,000 bins) are checked
+  against the oracle's order fed the GPU's first order.  The dense oracle
+  cannot hold this scene (F~ alone is 51 GB), so it runs on the sampled
+  pairs (oracle.p2p_visibility_pairs, fft_pairs, exchange_rows; pinned
+  against the dense functions in test_oracle_subsets.py).
+* Directional shoebox (1,072 patches, one BRDF per patch, 64 directions,
+  6 bands, 6,000 bins): the whole bake against the oracle's, and the same
+  per-order check on its plan (diffuse + specular).
+* Inverse materials on the directional shoebox: the GPU gradient of the
+  log-energy loss with respect to all 1,072 x 6 absorption and 1,072 x 6
+  scattering coefficients, at the scene's full patches, directions and
+  bands, against the FP64 reference gradient (oracle/adjoint.py, pinned by
+  finite differences), with the histogram cut to 160 bins (every pair's
+  delay, at most 153 bins, still fits) and 6 orders checkpointed every 5 so the dense oracle
+  fits in memory.
+* Batched rooms: the ten rooms of the seeded 256-room set that the FP64
+  oracle runs fastest, every (room, source) through BatchedExchange at the
+  set's full settings (order 20, 2,000 bins), each against the oracle
+  pipeline's receiver histograms.
+
+Tolerances as in test_gpu_parity.py: flags, indices and delays bit-exact,
+form factors 1e-9 relative, histograms 1e-5 relative within 60 dB of each
+histogram's peak.
+"""
+import numpy as np
+import pytest
+
+import oracle as orc
+from conftest import make_oracle, rel_err_within_db
+from sparrowpy_amd import scenes
+from sparrowpy_amd.coordinates import Coordinates, FrequencyData
+
+pytestmark = pytest.mark.gpu
+
+EXCHANGE_RTOL = 1e-5
+N_DEST = 32
+
+
+@pytest.fixture(scope="module")
+def torch():
+    import torch as _torch
+    if not _torch.cuda.is_available():
+        pytest.skip("no GPU")
+    return _torch
+
+
+def _log(msg):
+    print(f"[configs] {msg}", flush=True)
+
+
+def gpu_scene(sc):
+    from sparrowpy_amd import DirectionalRadiosityFast
+    r = DirectionalRadiosityFast.from_polygon(sc.walls, sc.patch_size)
+    for idx, br in sc.brdf_groups:
+        r.set_wall_brdf(idx, br, sc.directions, sc.directions)
+    r.set_air_attenuation(FrequencyData(sc.air_attenuation, sc.frequencies))
+    r.bake_geometry()
+    return r
+
+
+def oracle_scene(sc):
+    o = make_oracle(sc.walls, sc.patch_size)
+    for idx, br in sc.brdf_groups:
+        o.set_wall_brdf(np.atleast_1d(idx), br.freq, sc.directions.cartesian,
+                        sc.directions.cartesian, sc.frequencies)
+    o.set_air_attenuation(sc.air_attenuation, sc.frequencies)
+    return o
+
+
+def _norm_rows(v):
+    return np.sqrt((v[..., 0] * v[..., 0] + v[..., 1] * v[..., 1])
+                   + v[..., 2] * v[..., 2])
+
+
+def _plan_entries(plan, D, j):
+    """(sources, directions, delays) of destination j in the GPU plan, in
+    the plan's order."""
+    from sparrowpy_amd import ops
+    e0, e1 = int(plan.row_ptr[j]), int(plan.row_ptr[j + 1])
+    lag = plan.lag[e0:e1].to("cpu").numpy().astype(np.int64)
+    if isinstance(plan, ops.GroupedPlan):
+        rowid = plan.rowid[e0:e1].to("cpu").numpy().astype(np.int64)
+        return rowid // D, rowid % D, lag & (ops.GroupedPlan.LAST_IN_GROUP
+                                             - 1)
+    return (plan.src[e0:e1].cpu().numpy().astype(np.int64),
+            plan.dir[e0:e1].cpu().numpy().astype(np.int64), lag)
+
+
+def sampled_order_check(torch, r, o, sc, plan, E1, E2, dests, label):
+    """Bit-exact visibility, direction map, plan rows and delays for the
+    destinations ``dests``, then their order histograms E2 (GPU, computed
+    from the GPU's E1) against the oracle's order fed the same E1."""
+    P, D, B = r.n_patches, sc.directions.cartesian.shape[0], \
+        len(sc.frequencies)
+    T, c, dt = sc.n_samples, sc.speed_of_sound, sc.time_resolution
+    ctr, nrm, pts = o.patches_center, o.patches_normal, o.patches_points
+    wall = o.patch_to_wall_ids
+    vis_dev, p2o_dev = r._bake["vis"], r._bake["p2o"]
+    # 1. visibility of each destination against every other patch
+    others = [np.delete(np.arange(P), j) for j in dests]
+    lo = np.concatenate([np.minimum(i, j) for i, j in zip(others, dests)])
+    hi = np.concatenate([np.maximum(i, j) for i, j in zip(others, dests)])
+    vis_o = orc.p2p_visibility_pairs(ctr, nrm, pts, np.stack([lo, hi], -1))
+    lo_t = torch.as_tensor(lo, device=vis_dev.device)
+    hi_t = torch.as_tensor(hi, device=vis_dev.device)
+    vis_g = vis_dev[lo_t, hi_t].cpu().numpy().astype(bool)
+    np.testing.assert_array_equal(vis_g, vis_o)
+    _log(f"{label}: visibility of {len(dests)} patches against all "
+         f"{P - 1} others bit-exact ({int(vis_o.sum())} visible of "
+         f"{vis_o.size})")
+    # 2. per destination: sources, directions, delays, coefficients
+    splits = np.cumsum([len(x) for x in others])[:-1]
+    srcs = [i[v] for i, v in zip(others, np.split(vis_o, splits))]
+    pairs_up, entries = [], []
+    for j, s in zip(dests, srcs):
+        p2o_o = orc.receiver_index(ctr[s], ctr[j], o.brdf_out, wall[s])
+        col = p2o_dev[:, int(j)].cpu().numpy()
+        np.testing.assert_array_equal(col[s], p2o_o)
+        hidden = np.setdiff1d(np.delete(np.arange(P), j), s)
+        assert (col[hidden] == D).all()
+        lag_o = (_norm_rows(ctr[s] - ctr[j]) / c / dt).astype(np.int64)
+        gs, gd, gl = _plan_entries(plan, D, int(j))
+        order = np.argsort(gs, kind="stable")
+        np.testing.assert_array_equal(gs[order], s)
+        np.testing.assert_array_equal(gd[order], p2o_o)
+        np.testing.assert_array_equal(gl[order], lag_o)
+        entries.append((s, p2o_o, lag_o))
+        pairs_up.append(np.stack([np.minimum(s, j), np.maximum(s, j)], -1))
+    _log(f"{label}: direction map, plan source lists and delays of "
+         f"{sum(len(s) for s in srcs)} entries bit-exact")
+    # form factors of the pairs (i < j orientation), then F~ per entry
+    up = np.concatenate(pairs_up).astype(np.int32)
+    ffd = orc.form_factors(pts, nrm, o.patches_area, up)
+    dpairs = np.concatenate([np.stack([s, np.full_like(s, j)], -1)
+                             for j, (s, _, _) in zip(dests, entries)])
+    ffu = ffd[np.minimum(dpairs[:, 0], dpairs[:, 1]),
+              np.maximum(dpairs[:, 0], dpairs[:, 1])]
+    coef = orc.fft_pairs(dpairs, ffu, B, ctr, o.patches_area, o.air, wall,
+                         np.array(o.brdf), o.brdf_index, o.brdf_in, D)
+    # compact source rows (i, p2o[i, j]) shared by the destinations
+    keys = np.concatenate([s * D + d for s, d, _ in entries])
+    uniq, row_src = np.unique(keys, return_inverse=True)
+    row_ptr = np.concatenate([[0], np.cumsum([len(s) for s, _, _ in
+                                              entries])])
+    lag = np.concatenate([g for _, _, g in entries])
+    E1v = E1.view(P * D, B, T)
+    idx = torch.as_tensor(uniq, device=E1.device)
+    dt_ = torch.as_tensor(np.asarray(dests), device=E1.device)
+    worst = 0.0
+    for b in range(B):
+        prev_rows = E1v[idx, b].double().cpu().numpy()[:, None, :]
+        ref = orc.exchange_rows(T, row_ptr, row_src, lag,
+                                np.ascontiguousarray(coef[:, :, b:b + 1]),
+                                prev_rows)[:, :, 0, :]
+        out = E2[dt_, :, b, :].double().cpu().numpy()
+        err = rel_err_within_db(out, ref)
+        worst = max(worst, err)
+        assert ref.max() > 0
+        _log(f"{label}: band {b}: order-2 histograms of {len(dests)} "
+             f"destinations x {D} directions x {T} bins, max rel err "
+             f"{err:.2e}")
+        del prev_rows
+    assert worst <= EXCHANGE_RTOL, worst
+    return worst
+
+
+def two_orders(torch, r, sc, kind=None):
+    from sparrowpy_amd import ops
+    c, dt = sc.speed_of_sound, sc.time_resolution
+    plan = r._exchange_plan(c, dt, kind)
+    e0d = r._dev["energy_init_source"]
+    d0 = r._dev["distance_patches_to_source"]
+    P, D, B = e0d.shape
+    st = ops.ExchangeState(P, D, B, sc.n_samples)
+    st.init(e0d, d0, c, dt)
+    st.step(plan)
+    st.step(plan)
+    st.check()
+    torch.cuda.synchronize()
+    # after two steps prev holds order 2 and cur order 1
+    return plan, st
+
+
+def _injection_check(r, o, sc, label):
+    src = sc.sources[0]
+    r.init_source_energy(Coordinates(*src))
+    o.init_source_energy(src)
+    np.testing.assert_allclose(r._energy_init_source, o.energy_init_source,
+                               rtol=1e-9, atol=0)
+    np.testing.assert_allclose(r._distance_patches_to_source,
+                               o.distance_patches_to_source, rtol=0,
+                               atol=4 * np.spacing(
+                                   o.distance_patches_to_source.max()))
+    _log(f"{label}: injection of {r.n_patches} patches within 1e-9")
+
+
+def test_street_canyon_full_size(torch):
+    from sparrowpy_amd import ops
+    sc = scenes.street_canyon()
+    _log("street canyon: bake")
+    r = gpu_scene(sc)
+    o = oracle_scene(sc)
+    assert r.n_patches == 5122
+    _injection_check(r, o, sc, "canyon")
+    plan, st = two_orders(torch, r, sc)
+    assert isinstance(plan, ops.GroupedPlan) and plan.tiles is not None
+    _log(f"canyon: two orders on the {plan.tiles.JT}-row tiles of the "
+         f"grouped plan ({plan.Vd} directed pairs)")
+    rng = np.random.default_rng(2026)
+    dests = np.sort(rng.choice(r.n_patches, N_DEST, replace=False))
+    sampled_order_check(torch, r, o, sc, plan, st.cur, st.prev, dests,
+                        "canyon")
+    del st, plan
+    torch.cuda.empty_cache()
+
+
+def test_directional_shoebox_full_size(torch):
+    sc = scenes.directional_shoebox()
+    _log("directional shoebox: bake")
+    r = gpu_scene(sc)
+    o = oracle_scene(sc)
+    assert r.n_patches == 1072
+    # the whole bake against the oracle's
+    ctr = o.patches_center
+    vis_o = orc.p2p_visibility(ctr, o.patches_normal, o.patches_points)
+    np.testing.assert_array_equal(r._visibility_matrix, vis_o)
+    ii, jj = np.nonzero(vis_o)
+    pairs = np.stack([ii, jj], -1).astype(np.int32)
+    np.testing.assert_array_equal(r._visible_patches, pairs)
+    ff_o = orc.form_factors(o.patches_points, o.patches_normal,
+                            o.patches_area, pairs)
+    np.testing.assert_allclose(r._form_factors, ff_o, rtol=1e-9,
+                               atol=1e-15)
+    D = sc.directions.cartesian.shape[0]
+    p2o_g = r._patch_2_brdf_outgoing_index
+    vv = vis_o | vis_o.T
+    for j in range(r.n_patches):
+        s = np.nonzero(vv[:, j])[0]
+        np.testing.assert_array_equal(
+            p2o_g[s, j], orc.receiver_index(ctr[s], ctr[j], o.brdf_out,
+                                            o.patch_to_wall_ids[s]))
+        assert (np.delete(p2o_g[:, j], s) == D).all()
+    _log(f"shoebox: full bake bit-exact ({len(pairs)} visible pairs), "
+         "form factors within 1e-9")
+    _injection_check(r, o, sc, "shoebox")
+    plan, st = two_orders(torch, r, sc)
+    assert plan.diffuse_specular
+    rng = np.random.default_rng(2027)
+    dests = np.sort(rng.choice(r.n_patches, N_DEST, replace=False))
+    sampled_order_check(torch, r, o, sc, plan, st.cur, st.prev, dests,
+                        "shoebox")
+    del st, plan
+    torch.cuda.empty_cache()
+
+
+def _cheapest_rooms(scs, n):
+    """Indices of the n rooms whose FP64 oracle exchange is cheapest
+    (order count x pairs x bins x directions x bands x sources)."""
+    from sparrowpy_amd import geometry
+    cost = []
+    for x, sc in enumerate(scs):
+        wp = np.array([w.pts for w in sc.walls])
+        wn = np.array([w.normal for w in sc.walls])
+        _, _, P, _ = geometry.process_patches(wp, wn, sc.patch_size)
+        cost.append(P * P * sc.directions.cartesian.shape[0]
+                    * len(sc.frequencies) * len(sc.sources))
+    return list(np.argsort(cost, kind="stable")[:n])
+
+
+def test_batched_rooms_against_oracle(torch):
+    from sparrowpy_amd.batch import BatchedExchange, BatchItem
+    scs = scenes.batched_shoeboxes(256)
+    pick = _cheapest_rooms(scs, 10)
+    items = []
+    for x in pick:
+        sc = scs[x]
+        items.append(BatchItem(gpu_scene(sc), sc.sources, sc.receivers))
+    sc0 = scs[pick[0]]
+    c, dt, dur, K = (sc0.speed_of_sound, sc0.time_resolution, sc0.duration,
+                     sc0.max_order)
+    be = BatchedExchange(items, c, dt, dur, K)
+    out = {k: v.double().cpu().numpy() for k, v in be.run().items()}
+    _log(f"batched: {len(pick)} rooms, {len(out)} instances, classes "
+         f"{be.choice}")
+    worst = 0.0
+    for n, x in enumerate(pick):
+        sc = scs[x]
+        o = oracle_scene(sc)
+        o.bake_geometry()
+        for s, src in enumerate(sc.sources):
+            o.init_source_energy(src)
+            o.calculate_energy_exchange(c, dt, dur, K,
+                                        nthreads=orc.default_threads())
+            ref = o.collect_energy_receiver_mono(sc.receivers)
+            err = rel_err_within_db(out[(n, s)], ref)
+            worst = max(worst, err)
+            _log(f"batched: room {x} (P={o.n_patches}, "
+                 f"D={sc.directions.cartesian.shape[0]}, "
+                 f"B={len(sc.frequencies)}) source {s}: max rel err "
+                 f"{err:.2e}")
+            assert err <= EXCHANGE_RTOL, (x, s, err)
+    _log(f"batched: worst {worst:.2e}")
+
+
+def test_inverse_materials_gradient_full_size(torch):
+    from oracle import adjoint as oadj
+    from test_oracle_adjoint import loss_and_grad, spec_info
+    from sparrowpy_amd import DirectionalRadiosityFast, adjoint
+    from sparrowpy_amd import brdf as Bm
+    sc = scenes.directional_shoebox()
+    ta, ts = sc.materials
+    rng = np.random.default_rng(11)
+    alpha = np.clip(ta + rng.normal(0, .05, ta.shape), .01, .99)
+    scat = np.clip(ts + rng.normal(0, .1, ts.shape), .01, .99)
+    T, K, every = 160, 6, 5
+    c, dt, freqs = sc.speed_of_sound, sc.time_resolution, sc.frequencies
+    src, rcv = sc.sources[0], sc.receivers
+    dirs = sc.directions
+
+    def oracle_at(a, s):
+        o = make_oracle(sc.walls, sc.patch_size)
+        for n in range(len(sc.walls)):
+            br = Bm.create_from_scattering(
+                dirs, dirs, FrequencyData(s[n][None, :], freqs),
+                FrequencyData(a[n][None, :], freqs))
+            o.set_wall_brdf([n], br.freq, dirs.cartesian, dirs.cartesian,
+                            freqs)
+        o.set_air_attenuation(sc.air_attenuation, freqs)
+        o.bake_geometry()
+        o.init_source_energy(src)
+        o.calculate_energy_exchange(c, dt, T * dt, K,
+                                    nthreads=orc.default_threads())
+        return o, o.collect_energy_receiver_mono(rcv)
+
+    _, target = oracle_at(ta, ts)
+    ro, Ho = oracle_at(alpha, scat)
+    _log(f"inverse: oracle forward at P={ro.n_patches}, T={T}, K={K}")
+    L, gH = loss_and_grad(Ho, target)
+    sv, m = spec_info(dirs)
+    ga_ref, gs_ref = oadj.scattering_gradient(
+        oadj.gradient(ro, rcv, gH, K), alpha, scat, sv, m)
+    del ro
+    _log("inverse: FP64 reference gradient done")
+    r = DirectionalRadiosityFast.from_polygon(sc.walls, sc.patch_size)
+    for n in range(len(sc.walls)):
+        br = Bm.create_from_scattering(
+            dirs, dirs, FrequencyData(scat[n][None, :], freqs),
+            FrequencyData(alpha[n][None, :], freqs))
+        r.set_wall_brdf([n], br, dirs, dirs)
+    r.set_air_attenuation(FrequencyData(sc.air_attenuation, freqs))
+    r.bake_geometry()
+    adj = adjoint.ExchangeAdjoint(r, Coordinates(*src),
+                                  Coordinates(*rcv.T), c, dt, T * dt, K,
+                                  checkpoint_every=every)
+    kw = dict(dtype=torch.float64, device="cuda")
+    a = torch.tensor(alpha, requires_grad=True, **kw)
+    s = torch.tensor(scat, requires_grad=True, **kw)
+    A, C = adjoint.scattering_tables(a, s, sv)
+    loss = adjoint.log_energy_loss(adjoint.receiver_histograms(adj, A, C),
+                                   target)
+    loss.backward()
+    ga, gs = a.grad.cpu().numpy(), s.grad.cpu().numpy()
+    gmax = max(np.abs(ga_ref).max(), np.abs(gs_ref).max())
+    err = max(np.abs(ga - ga_ref).max(), np.abs(gs - gs_ref).max()) / gmax
+    _log(f"inverse: {alpha.size + scat.size} parameters, loss gpu "
+         f"{float(loss):.6e} fp64 {L:.6e}, max grad err / max grad "
+         f"{err:.2e}")
+    assert abs(float(loss) - L) <= 1e-5 * L
+    assert err <= 1e-4
