@@ -379,16 +379,20 @@ int spr_exchange_gather_ds(int64_t P, int64_t B, int64_t T, int64_t Vd,
                            const float *prev, float *gsum, void *stream);
 
 /* Destination tiles of the grouped plan (the _energy_exchange loop,
- * RadiosityFast.py:1073-1145, as grouped above).  A tile is JT (4 or 8)
- * consecutive destination rows; its entries are merged into distinct
- * (group key, source row) entries, in key then source-row order, each with
- * one delay word and one weight per slot (row of the tile), so the JT
- * gathers of one source row are issued together and share cache lines.
+ * RadiosityFast.py:1073-1145, as grouped above).  A tile is JT (2..16)
+ * consecutive rows of a plan (a caller may permute and pad the rows so
+ * that a tile is a block of neighbouring patches); its entries are merged
+ * into distinct (key, source row) entries, in key then source-row order,
+ * each with one delay word and one weight per slot (row of the tile), so
+ * the JT gathers of one source row share one load.
  *
  * Pass 1: skey[Vd], sorted[Vd], payload[Vd] (int64 workspace), first[Vd+1]
  * (int64: exclusive count of distinct entries before each merged position;
  * first[Vd] = M, the number of merged entries) and tptr[ceil(P/JT)+1]
- * (merged CSR of the tiles). */
+ * (merged CSR of the tiles).  skey[e] = (key << 32) | rowid[e] is computed
+ * from the groups (grp_start, grp_key), or, when both are NULL, given by
+ * the caller (the diffuse + specular plan keys entries by their own
+ * specular direction). */
 int spr_plan_tile_order(int64_t P, int64_t JT, int64_t Vd, int64_t G,
                         const int64_t *row_ptr, const int64_t *grp_start,
                         const int32_t *grp_key, const int32_t *rowid,
@@ -410,7 +414,12 @@ int spr_plan_tile_fill(int64_t P, int64_t JT, int64_t B, int64_t Vd,
 
 /* Gather pass on the tiles: gsum bit-identical to
  * spr_exchange_gather_grouped (each slot sums its groups in the same
- * order), for use with spr_exchange_spread_grouped.
+ * order), for use with spr_exchange_spread_grouped.  JT = 4 or 8: the
+ * slots' loads of a merged source row are issued together and share cache
+ * lines; JT = 16: one wave per slot, each merged row's window staged once
+ * in LDS.  grp_ptr[slot] = index of the slot's first group (the plan's
+ * grp_ptr for consecutive rows; for permuted rows, that of the slot's
+ * destination); P = number of slots.
  * Replaces, with a spread, one order of _energy_exchange,
  * RadiosityFast.py:1121-1144. */
 int spr_exchange_gather_tiled(int64_t P, int64_t JT, int64_t B, int64_t T,
@@ -418,6 +427,23 @@ int spr_exchange_gather_tiled(int64_t P, int64_t JT, int64_t B, int64_t T,
                               const int64_t *grp_ptr, const int32_t *mrow,
                               const int32_t *mlag, const double *mw,
                               const float *prev, float *gsum, void *stream);
+
+/* Gather pass of the diffuse + specular plan on 16-row tiles (merged by
+ * each entry's specular direction): gsum bit-identical to
+ * spr_exchange_gather_ds, for use with spr_exchange_spread_ds.  mw and
+ * mdw[B,M,16]: specular and diffuse weights per slot (two
+ * spr_plan_tile_fill passes, with scal and with dscal); gstart[slot],
+ * gdiff[slot] (int64): the slot's first group and its diffuse group (-1:
+ * the slot's row has no entries).
+ * Replaces, with spr_exchange_spread_ds, one order of _energy_exchange,
+ * RadiosityFast.py:1121-1144. */
+int spr_exchange_gather_tiled_ds(int64_t P, int64_t JT, int64_t B, int64_t T,
+                                 int64_t M, const int64_t *tptr,
+                                 const int64_t *gstart, const int64_t *gdiff,
+                                 const int32_t *mrow, const int32_t *mlag,
+                                 const double *mw, const double *mdw,
+                                 const float *prev, float *gsum,
+                                 void *stream);
 
 /* etc[P,D,B,T] (float32) = tot_hi + tot_lo (may alias tot_hi).
  * The total histogram _energy_exchange returns (RadiosityFast.py:1114-1145). */

@@ -226,8 +226,9 @@ class BatchedExchange:
             [[0], np.cumsum([self.items[x].radiosity.n_patches
                              for x in xs])]).astype(np.int64)
         plan = _concat_plans([plans[x] for x in xs], offsets[:-1])
-        if (self.tile_rows and isinstance(plan, ops.GroupedPlan) and
-                not plan.diffuse_specular):
+        if self.tile_rows and isinstance(plan, ops.GroupedPlan) and (
+                not plan.diffuse_specular or self.tile_rows == 16):
+            # diffuse + specular plans tile at 16 rows only
             plan.tiles = ops.tile_plan(plan, self.tile_rows)
         e0d, d0 = [], []
         for x, s in chunk:

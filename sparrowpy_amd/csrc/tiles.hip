@@ -343,6 +343,13 @@ int launch_tile_shape(int64_t P, int64_t B, int64_t T, int64_t M,
   return SPR_OK;
 }
 
+template <int R, int JT, int NB, int SMAX, bool DS>
+int launch_tile_lds(int64_t P, int64_t B, int64_t T, int64_t M,
+                    const int64_t *tptr, const int64_t *grp_ptr,
+                    const int64_t *gdiff, const int32_t *mrow,
+                    const int32_t *mlag, const double *mw, const double *mdw,
+                    const float *prev, float *gsum, hipStream_t s);
+
 int launch_tile_gather(int64_t P, int JT, int64_t B, int64_t T, int64_t M,
                        const int64_t *tptr, const int64_t *grp_ptr,
                        const int32_t *mrow, const int32_t *mlag,
@@ -351,8 +358,259 @@ int launch_tile_gather(int64_t P, int JT, int64_t B, int64_t T, int64_t M,
   if (JT == 4)
     return launch_tile_shape<4, 1, 4, 4, 64>(P, B, T, M, tptr, grp_ptr, mrow,
                                              mlag, mw, prev, gsum, s);
-  return launch_tile_shape<4, 1, 8, 4, 64>(P, B, T, M, tptr, grp_ptr, mrow,
-                                           mlag, mw, prev, gsum, s);
+  if (JT == 8)
+    return launch_tile_shape<4, 1, 8, 4, 64>(P, B, T, M, tptr, grp_ptr,
+                                             mrow, mlag, mw, prev, gsum, s);
+  return launch_tile_lds<16, 16, 16, 64, false>(P, B, T, M, tptr, grp_ptr,
+                                                nullptr, mrow, mlag, mw,
+                                                nullptr, prev, gsum, s);
+}
+
+// ---------------------------------------------------------------------
+// the LDS-staged tiled gather (16-row tiles)
+// ---------------------------------------------------------------------
+// With 16 rows per tile a merged entry serves several slots, but loading
+// its source row once per slot through the caches (as tile_gather_kernel
+// does for 4 rows) costs one cache read per slot and, across many waves,
+// evicts the lines before the last slot reads them.  Here the workgroup
+// stages each merged entry's window of the source row in LDS once, and
+// every slot (one wave per destination row of the tile) reads its delayed
+// view of it from LDS.
+//
+// A workgroup owns (tile, band, block of TB = 64*R bins).  Merged entries
+// are taken in batches of NB; for each, the window of source bins
+// [t0 - maxlag, t0 - maxlag + W) with W = TB + SMAX covers every present
+// slot whose delay is within SMAX of the entry's largest delay (maxlag); a
+// slot with a larger spread reads its bins from global memory instead
+// (uniform branch, rare).  Two LDS buffers: the global loads of batch q+1
+// are in flight while batch q is summed from LDS; one barrier per batch.
+// Each wave sums its slot's entries in merged order, which within a group
+// is the grouped plan's entry order: the per-group sums are bit-identical
+// to spr_exchange_gather_grouped's.  Requires T < 2^29.
+template <int JT>
+struct SlotMeta {
+  int32_t row;     // source row of the merged entry
+  int32_t lag;     // this wave's slot delay word (kLagMask: absent)
+  int32_t maxlag;  // largest delay of the entry's present slots
+  int32_t minlag;  // smallest
+  double w;        // this wave's slot weight in the workgroup's band
+  double dw;       // its diffuse weight (diffuse + specular plan)
+};
+
+// Merged entries m0 .. m0+63, one per lane, as seen by slot k.
+template <int JT>
+__device__ __forceinline__ SlotMeta<JT> load_slot_meta(
+    int64_t m0, int64_t m_end, int k, const int32_t *__restrict__ mrow,
+    const int32_t *__restrict__ mlag, const double *__restrict__ w,
+    const double *__restrict__ dw) {
+  const int64_t m = m0 + (int64_t)(threadIdx.x & 63);
+  const bool in = m < m_end;
+  const int64_t mc = in ? m : m_end - 1;
+  SlotMeta<JT> t;
+  t.row = mrow[mc];
+  int32_t mx = 0, mn = kLagMask;
+  const int4 *lw = reinterpret_cast<const int4 *>(mlag + mc * JT);
+#pragma unroll
+  for (int q = 0; q < JT / 4; ++q) {
+    const int4 v = lw[q];
+    const int32_t a[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int32_t l = a[u] & kLagMask;
+      if (l != kLagMask) {
+        mx = max(mx, l);
+        mn = min(mn, l);
+      }
+    }
+  }
+  t.maxlag = mx;
+  t.minlag = mn == kLagMask ? mx : mn;
+  t.lag = in ? mlag[mc * JT + k] : kLagMask;
+  t.w = in ? w[mc * JT + k] : 0.0;
+  t.dw = in && dw ? dw[mc * JT + k] : 0.0;
+  return t;
+}
+
+template <int R, int JT, int NB, int SMAX, bool DS>
+__global__ __launch_bounds__(64 * JT) void tile_lds_gather_kernel(
+    int64_t P, int B, int T, int n_tblk, int64_t nblocks, int64_t M,
+    const int64_t *__restrict__ tptr, const int64_t *__restrict__ grp_ptr,
+    const int64_t *__restrict__ gdiff, const int32_t *__restrict__ mrow,
+    const int32_t *__restrict__ mlag, const double *__restrict__ mw,
+    const double *__restrict__ mdw, const float *__restrict__ prev,
+    float *__restrict__ gsum) {
+  constexpr int TB = 64 * R;       // bins of the workgroup
+  constexpr int W = TB + SMAX;     // LDS window of one merged entry
+  constexpr int NX = W / 64;       // window floats per lane
+  constexpr int NE = NB / JT;      // entries each wave stages per batch
+  static_assert(SMAX % 64 == 0 && NB % JT == 0 && 64 % NB == 0, "shape");
+  __shared__ float lds[2][NB][W];
+  int64_t tile;
+  int b, tb;
+  dest_major_tile(xcd_tile(blockIdx.x, nblocks), nblocks / (B * n_tblk),
+                  n_tblk, tile, b, tb);
+  const int64_t m_begin = tptr[tile], m_end = tptr[tile + 1];
+  if (m_end <= m_begin) return;  // uniform: before any barrier
+  const int t0 = tb * TB;
+  const int k = (int)(threadIdx.x >> 6);   // slot = wave
+  const int lane = (int)(threadIdx.x & 63);
+  const int64_t j = tile * JT + k;
+  const bool active = j < P;
+  int64_t g = active ? grp_ptr[j] : 0;
+  const double *w = mw + (int64_t)b * M * JT;
+  const double *dw = DS ? mdw + (int64_t)b * M * JT : nullptr;
+  const uint32_t row_bytes = (uint32_t)T * 4u;
+  const int64_t bstride = (int64_t)B * T;
+
+  // batch staging: wave k loads entries k, k + JT, ... of the batch
+  float st[NE][NX];
+  auto stage_load = [&](const SlotMeta<JT> &m, int64_t mb, int64_t blk0) {
+#pragma unroll
+    for (int e = 0; e < NE; ++e) {
+      const int64_t mm = mb + k + e * JT;
+      const int idx = (int)(mm - blk0);
+      const bool have = mm < m_end;
+      const int32_t row = __builtin_amdgcn_readlane(m.row, have ? idx : 0);
+      const int32_t mx = __builtin_amdgcn_readlane(m.maxlag, have ? idx : 0);
+      const int32_t mn = __builtin_amdgcn_readlane(m.minlag, have ? idx : 0);
+      const int need = have ? min(W, TB + (mx - mn)) : 0;
+      const __amdgpu_buffer_rsrc_t rs =
+          make_rsrc(prev + (int64_t)row * bstride + (int64_t)b * T,
+                    row_bytes);
+      const int s0 = t0 - mx + lane;
+#pragma unroll
+      for (int x = 0; x < NX; ++x)
+        st[e][x] = 64 * x < need ? buffer_load(rs, bin_offset(s0 + 64 * x))
+                                 : 0.0f;
+    }
+  };
+  auto stage_write = [&](int buf) {
+#pragma unroll
+    for (int e = 0; e < NE; ++e)
+#pragma unroll
+      for (int x = 0; x < NX; ++x)
+        lds[buf][k + e * JT][lane + 64 * x] = st[e][x];
+  };
+
+  double S[R], Sd[DS ? R : 1];
+#pragma unroll
+  for (int r = 0; r < R; ++r) S[r] = 0.0;
+#pragma unroll
+  for (int r = 0; r < (DS ? R : 1); ++r) Sd[r] = 0.0;
+  auto compute = [&](const SlotMeta<JT> &m, int64_t mb, int64_t blk0,
+                     int buf) {
+#pragma unroll
+    for (int e = 0; e < NB; ++e) {
+      const int64_t mm = mb + e;
+      if (mm >= m_end) break;  // uniform
+      const int idx = (int)(mm - blk0);
+      const int32_t lw = __builtin_amdgcn_readlane(m.lag, idx);
+      const int32_t l = lw & kLagMask;
+      if (active && l != kLagMask) {
+        const double wk = readlane_f64(m.w, idx);
+        const int32_t mx = __builtin_amdgcn_readlane(m.maxlag, idx);
+        const int off = mx - l;
+        float v[R];
+        if (off <= SMAX) {
+#pragma unroll
+          for (int r = 0; r < R; ++r) v[r] = lds[buf][e][off + lane + 64 * r];
+        } else {  // delay far from the entry's others: global memory
+          const int32_t row = __builtin_amdgcn_readlane(m.row, idx);
+          const __amdgpu_buffer_rsrc_t rs =
+              make_rsrc(prev + (int64_t)row * bstride + (int64_t)b * T,
+                        row_bytes);
+#pragma unroll
+          for (int r = 0; r < R; ++r)
+            v[r] = buffer_load(rs, bin_offset(t0 + lane + 64 * r - l));
+        }
+        if (DS) {
+          const double dk = readlane_f64(m.dw, idx);
+#pragma unroll
+          for (int r = 0; r < R; ++r) {
+            const double x = (double)v[r];
+            S[r] = fma(wk, x, S[r]);
+            Sd[DS ? r : 0] = fma(dk, x, Sd[DS ? r : 0]);
+          }
+        } else {
+#pragma unroll
+          for (int r = 0; r < R; ++r) S[r] = fma(wk, (double)v[r], S[r]);
+        }
+      }
+      if (active && (lw & kLastInGroup)) {
+        const __amdgpu_buffer_rsrc_t os =
+            make_rsrc(gsum + (g * B + b) * (int64_t)T, row_bytes);
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          buffer_store((float)S[r], os, (uint32_t)(t0 + lane + 64 * r) * 4u);
+          S[r] = 0.0;
+        }
+        g += 1;
+      }
+    }
+  };
+
+  SlotMeta<JT> cur =
+      load_slot_meta<JT>(m_begin, m_end, k, mrow, mlag, w, dw);
+  SlotMeta<JT> nxt =
+      load_slot_meta<JT>(m_begin + 64, m_end, k, mrow, mlag, w, dw);
+  int64_t blk0 = m_begin;  // first entry of the metadata block in `cur`
+  stage_load(cur, m_begin, blk0);
+  stage_write(0);
+  __syncthreads();
+  int buf = 0;
+  for (int64_t mb = m_begin; mb < m_end; mb += NB) {
+    const int64_t mb1 = mb + NB;
+    const bool more = mb1 < m_end;
+    const bool new_blk = mb1 - blk0 >= 64;
+    if (more) {
+      if (new_blk)
+        stage_load(nxt, mb1, blk0 + 64);
+      else
+        stage_load(cur, mb1, blk0);
+    }
+    compute(cur, mb, blk0, buf);
+    if (more) stage_write(buf ^ 1);
+    __syncthreads();
+    buf ^= 1;
+    if (new_blk) {
+      cur = nxt;
+      blk0 += 64;
+      nxt = load_slot_meta<JT>(blk0 + 64, m_end, k, mrow, mlag, w, dw);
+    }
+  }
+  // diffuse + specular plan: the row's diffuse sum over all its entries
+  // goes to its diffuse group (the row's last), as ds_gather_kernel stores
+  if (DS && active && gdiff[j] >= 0) {
+    const __amdgpu_buffer_rsrc_t os =
+        make_rsrc(gsum + (gdiff[j] * B + b) * (int64_t)T, row_bytes);
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+      buffer_store((float)Sd[DS ? r : 0], os,
+                   (uint32_t)(t0 + lane + 64 * r) * 4u);
+  }
+}
+
+// 16-row tiles: R = 16 bins per lane (1,024-bin blocks; 8 for the diffuse
+// + specular plan, whose second sums take the registers), batches of 16
+// merged entries, windows of TB + 64 bins (2 x 16 x 1,088 floats of LDS
+// at R = 16, one workgroup of 16 waves per CU).
+template <int R, int JT, int NB, int SMAX, bool DS>
+int launch_tile_lds(int64_t P, int64_t B, int64_t T, int64_t M,
+                    const int64_t *tptr, const int64_t *grp_ptr,
+                    const int64_t *gdiff, const int32_t *mrow,
+                    const int32_t *mlag, const double *mw, const double *mdw,
+                    const float *prev, float *gsum, hipStream_t s) {
+  const int64_t n_tiles = ceil_div(P, JT);
+  const int64_t nt = ceil_div(T, (int64_t)64 * R);
+  const int64_t nb = n_tiles * B * nt;
+  SPR_REQUIRE(nb < (int64_t)0x7fffffff, "exchange grid too large");
+  SPR_REQUIRE(T < ((int64_t)1 << 29), "histogram too long (T >= 2^29)");
+  hipLaunchKernelGGL((tile_lds_gather_kernel<R, JT, NB, SMAX, DS>),
+                     dim3((unsigned)nb), dim3(64 * JT), 0, s, P, (int)B,
+                     (int)T, (int)nt, nb, M, tptr, grp_ptr, gdiff, mrow,
+                     mlag, mw, mdw, prev, gsum);
+  SPR_LAUNCH_CHECK();
+  return SPR_OK;
 }
 
 unsigned grid_of(int64_t n) { return (unsigned)ceil_div(n > 0 ? n : 1, 256); }
@@ -371,19 +629,23 @@ extern "C" int spr_plan_tile_order(int64_t P, int64_t JT, int64_t Vd,
                                    int64_t *first, int64_t *tptr,
                                    void *stream) {
   clear_error();
-  SPR_REQUIRE(P > 0 && Vd >= 0 && G >= 0 && JT >= 2 && JT <= 8,
-              "spr_plan_tile_order: bad sizes (JT must be 2..8)");
+  SPR_REQUIRE(P > 0 && Vd >= 0 && G >= 0 && JT >= 2 && JT <= 16,
+              "spr_plan_tile_order: bad sizes (JT must be 2..16)");
   SPR_REQUIRE(Vd < ((int64_t)1 << 40), "spr_plan_tile_order: plan too large");
   SPR_REQUIRE(row_ptr && tptr && first, "spr_plan_tile_order: null pointer");
-  SPR_REQUIRE(Vd == 0 || (grp_start && grp_key && rowid && skey && sorted &&
-                          payload && G > 0),
+  const bool given = !grp_start && !grp_key;
+  SPR_REQUIRE(Vd == 0 || (skey && sorted && payload &&
+                          (given || (grp_start && grp_key && rowid &&
+                                     G > 0))),
               "spr_plan_tile_order: null pointer");
   hipStream_t s = as_stream(stream);
   const int64_t n_tiles = ceil_div(P, JT);
   if (Vd > 0) {
-    hipLaunchKernelGGL(tile_skey_kernel, dim3(grid_of(Vd)), dim3(256), 0, s,
-                       Vd, G, grp_start, grp_key, rowid, skey);
-    SPR_LAUNCH_CHECK();
+    if (!given) {
+      hipLaunchKernelGGL(tile_skey_kernel, dim3(grid_of(Vd)), dim3(256), 0,
+                         s, Vd, G, grp_start, grp_key, rowid, skey);
+      SPR_LAUNCH_CHECK();
+    }
     hipLaunchKernelGGL(tile_merge_kernel, dim3(grid_of(Vd)), dim3(256), 0, s,
                        P, (int)JT, Vd, row_ptr, skey, sorted, payload);
     SPR_LAUNCH_CHECK();
@@ -409,8 +671,8 @@ extern "C" int spr_plan_tile_fill(int64_t P, int64_t JT, int64_t B,
                                   int32_t *mlag, double *mw, void *stream) {
   clear_error();
   SPR_REQUIRE(P > 0 && B > 0 && Vd >= 0 && M >= 0 && M <= Vd && JT >= 2 &&
-                  JT <= 8,
-              "spr_plan_tile_fill: bad sizes (JT must be 2..8)");
+                  JT <= 16,
+              "spr_plan_tile_fill: bad sizes (JT must be 2..16)");
   if (M == 0) return SPR_OK;
   SPR_REQUIRE(row_ptr && rowid && lag && scal && sorted && payload && first &&
                   mrow && mlag && mw,
@@ -437,10 +699,26 @@ extern "C" int spr_exchange_gather_tiled(int64_t P, int64_t JT, int64_t B,
                                          const int32_t *mlag,
                                          const double *mw, const float *prev,
                                          float *gsum, void *stream) {
-  SPR_REQUIRE(P > 0 && B > 0 && T > 0 && M >= 0 && (JT == 4 || JT == 8),
-              "spr_exchange_gather_tiled: bad sizes (JT must be 4 or 8)");
+  SPR_REQUIRE(P > 0 && B > 0 && T > 0 && M >= 0 &&
+                  (JT == 4 || JT == 8 || JT == 16),
+              "spr_exchange_gather_tiled: bad sizes (JT must be 4, 8 or 16)");
   SPR_REQUIRE(tptr && grp_ptr && mrow && mlag && mw && prev && gsum,
               "spr_exchange_gather_tiled: null pointer");
   return launch_tile_gather(P, (int)JT, B, T, M, tptr, grp_ptr, mrow, mlag,
                             mw, prev, gsum, as_stream(stream));
+}
+
+extern "C" int spr_exchange_gather_tiled_ds(
+    int64_t P, int64_t JT, int64_t B, int64_t T, int64_t M,
+    const int64_t *tptr, const int64_t *gstart, const int64_t *gdiff,
+    const int32_t *mrow, const int32_t *mlag, const double *mw,
+    const double *mdw, const float *prev, float *gsum, void *stream) {
+  SPR_REQUIRE(P > 0 && B > 0 && T > 0 && M >= 0 && JT == 16,
+              "spr_exchange_gather_tiled_ds: bad sizes (JT must be 16)");
+  SPR_REQUIRE(tptr && gstart && gdiff && mrow && mlag && mw && mdw && prev &&
+                  gsum,
+              "spr_exchange_gather_tiled_ds: null pointer");
+  return launch_tile_lds<8, 16, 16, 64, true>(P, B, T, M, tptr, gstart,
+                                               gdiff, mrow, mlag, mw, mdw,
+                                               prev, gsum, as_stream(stream));
 }

@@ -445,36 +445,137 @@ class GroupedPlan:
 
 @dataclasses.dataclass
 class TilePlan:
-    """Destination tiles of a GroupedPlan: JT consecutive rows merged into
-    M distinct (group key, source row) entries with one delay word and one
-    weight per slot; ``tptr`` is the tiles' CSR over the merged entries,
-    ``mlag[M, JT]`` and ``mw[B, M, JT]`` (see sparrow_hip.h)."""
+    """Destination tiles of a GroupedPlan: JT rows merged into M distinct
+    (group key, source row) entries with one delay word and one weight per
+    slot; ``tptr`` is the tiles' CSR over the merged entries, ``mlag[M,
+    JT]`` and ``mw[B, M, JT]`` (see sparrow_hip.h).  ``P`` is the number of
+    slots (tiles x JT rows, empty rows included) and ``gstart[slot]`` the
+    index of the slot's first group in the plan (where its sums go)."""
 
     JT: int
     M: int
+    P: int
+    gstart: torch.Tensor
     tptr: torch.Tensor
     mrow: torch.Tensor
     mlag: torch.Tensor
     mw: torch.Tensor
+    # diffuse + specular plan: diffuse weights [B, M, JT] and each slot's
+    # diffuse group (-1: none)
+    mdw: torch.Tensor = None
+    gdiff: torch.Tensor = None
 
 
-def tile_plan(plan, jt=4, min_merge=1.5):
-    """TilePlan of a (non diffuse + specular) GroupedPlan, or None when
-    tiles of ``jt`` rows merge fewer than ``min_merge`` entries into one."""
-    if plan.diffuse_specular:
-        raise ValueError("tiles are built for the grouped plan only")
-    jt = int(jt)
-    P, B, Vd, G = plan.P, plan.B, plan.Vd, plan.n_groups
+def _rows_in_order(plan, order):
+    """The grouped plan's arrays with its destination rows in ``order``
+    (destination index per new row, -1 for an empty row): a dict of
+    row_ptr, grp_start, grp_key (grouped plan only), rowid, lag, scal,
+    dscal and entry_key (diffuse + specular plan only), gstart and gend.
+    Every row keeps its entries and groups in their order; gstart[r] and
+    gend[r] are the plan's indices of the first and one past the last group
+    of the row's destination (0 for an empty row)."""
     dev = plan.row_ptr.device
-    n_tiles = -(-P // jt)
+    order = torch.as_tensor(np.asarray(order, np.int64), device=dev)
+    real = order >= 0
+    zero = torch.zeros_like(order)
+    src = torch.where(real, order, zero)
+    rp, gp = plan.row_ptr, plan.grp_ptr
+    cnt = torch.where(real, rp[src + 1] - rp[src], zero)
+    row_ptr = torch.zeros(len(order) + 1, dtype=torch.int64, device=dev)
+    row_ptr[1:] = torch.cumsum(cnt, 0)
+    Vd = int(row_ptr[-1])
+    # entry e of new row r comes from rp[src[r]] + (e - row_ptr[r])
+    r_of_e = torch.repeat_interleave(torch.arange(len(order), device=dev),
+                                     cnt)
+    eidx = rp[src][r_of_e] + (torch.arange(Vd, device=dev) - row_ptr[r_of_e])
+    out = dict(row_ptr=row_ptr, rowid=plan.rowid[eidx].contiguous(),
+               lag=plan.lag[eidx].contiguous(),
+               scal=plan.scal[:, eidx].contiguous(),
+               gstart=torch.where(real, gp[src], zero),
+               gend=torch.where(real, gp[src + 1], zero))
+    if plan.diffuse_specular:
+        out["dscal"] = plan.dscal[:, eidx].contiguous()
+        out["entry_key"] = plan.entry_key[eidx].contiguous()
+        return out
+    gcnt = torch.where(real, gp[src + 1] - gp[src], zero)
+    g_of_r = torch.repeat_interleave(torch.arange(len(order), device=dev),
+                                     gcnt)
+    gofs = torch.zeros(len(order) + 1, dtype=torch.int64, device=dev)
+    gofs[1:] = torch.cumsum(gcnt, 0)
+    G = int(gofs[-1])
+    gidx = gp[src][g_of_r] + (torch.arange(G, device=dev) - gofs[g_of_r])
+    # group g of the plan starts at entry grp_start[g] (every group of the
+    # grouped plan has entries); in the new order at that entry's position
+    new_pos = torch.empty(plan.Vd, dtype=torch.int64, device=dev)
+    new_pos[eidx] = torch.arange(Vd, device=dev)
+    grp_start = torch.empty(G + 1, dtype=torch.int64, device=dev)
+    grp_start[:G] = new_pos[plan.grp_start[gidx]]
+    grp_start[G] = Vd
+    out["grp_start"] = grp_start
+    out["grp_key"] = plan.grp_key[gidx].contiguous()
+    return out
+
+
+def tile_plan(plan, jt=4, min_merge=1.5, order=None):
+    """TilePlan of a GroupedPlan, or None when tiles of ``jt`` rows merge
+    fewer than ``min_merge`` entries into one.
+
+    Grouped plans take ``jt`` of 4, 8 or 16 rows, diffuse + specular plans
+    16 (merged by each entry's specular direction; the tiled gather also
+    sums every row's diffuse group).  ``order`` (optional): the
+    destination of each slot, tile after tile (-1 for an empty slot), e.g.
+    from :func:`block_order`, so that a tile can be a compact block of
+    patches rather than ``jt`` consecutive ids.  Every destination must
+    appear once."""
+    ds = plan.diffuse_specular
+    jt = int(jt)
+    if ds and jt != 16:
+        raise ValueError("diffuse + specular tiles have 16 rows")
+    B = plan.B
+    dev = plan.row_ptr.device
+    if order is None:
+        order = np.arange(plan.P, dtype=np.int64)
+    else:
+        order = np.asarray(order, np.int64)
+        real = np.sort(order[order >= 0])
+        if len(real) != plan.P or not np.array_equal(real,
+                                                    np.arange(plan.P)):
+            raise ValueError("order must list every destination once")
+    if len(order) % jt:
+        order = np.concatenate([order, np.full(jt - len(order) % jt, -1)])
+    if not ds and len(order) == plan.P + (-plan.P) % jt and \
+            np.array_equal(order[:plan.P], np.arange(plan.P)):
+        # consecutive rows: the plan's own arrays
+        q = dict(row_ptr=torch.cat([plan.row_ptr, plan.row_ptr[-1:].expand(
+                     len(order) - plan.P)]),
+                 grp_start=plan.grp_start, grp_key=plan.grp_key,
+                 rowid=plan.rowid, lag=plan.lag, scal=plan.scal)
+        gstart = torch.cat([plan.grp_ptr[:-1], torch.zeros(
+            len(order) - plan.P, dtype=torch.int64, device=dev)])
+    else:
+        q = _rows_in_order(plan, order)
+        gstart = q["gstart"]
+    P, Vd = len(order), int(q["rowid"].numel())
+    n_tiles = P // jt
     tptr = torch.empty(n_tiles + 1, dtype=torch.int64, device=dev)
     first = torch.empty(Vd + 1, dtype=torch.int64, device=dev)
     work = [torch.empty(max(Vd, 1), dtype=torch.int64, device=dev)
             for _ in range(3)]
     skey, srt, payload = work
-    call("spr_plan_tile_order", P, jt, Vd, G, ptr(plan.row_ptr),
-         ptr(plan.grp_start), ptr(plan.grp_key), ptr(plan.rowid), ptr(skey),
-         ptr(srt), ptr(payload), ptr(first), ptr(tptr), stream_ptr())
+    if ds:
+        # merge key: the entry's own specular direction (entries without
+        # one have no group, so their group's key would be wrong)
+        skey[:Vd] = (q["entry_key"].to(torch.int64) << 32) | \
+            q["rowid"].to(torch.int64)
+        call("spr_plan_tile_order", P, jt, Vd, 0, ptr(q["row_ptr"]), None,
+             None, ptr(q["rowid"]), ptr(skey), ptr(srt), ptr(payload),
+             ptr(first), ptr(tptr), stream_ptr())
+    else:
+        G = int(q["grp_key"].numel())
+        call("spr_plan_tile_order", P, jt, Vd, G, ptr(q["row_ptr"]),
+             ptr(q["grp_start"]), ptr(q["grp_key"]), ptr(q["rowid"]),
+             ptr(skey), ptr(srt), ptr(payload), ptr(first), ptr(tptr),
+             stream_ptr())
     M = int(first[Vd].item())
     if M == 0 or Vd < min_merge * M:
         return None
@@ -482,11 +583,53 @@ def tile_plan(plan, jt=4, min_merge=1.5):
     mrow = torch.empty(M, dtype=torch.int32, device=dev)
     mlag = torch.empty((M, jt), dtype=torch.int32, device=dev)
     mw = torch.empty((B, M, jt), dtype=torch.float64, device=dev)
-    call("spr_plan_tile_fill", P, jt, B, Vd, M, ptr(plan.row_ptr),
-         ptr(plan.rowid), ptr(plan.lag), ptr(plan.scal), ptr(srt),
+    call("spr_plan_tile_fill", P, jt, B, Vd, M, ptr(q["row_ptr"]),
+         ptr(q["rowid"]), ptr(q["lag"]), ptr(q["scal"]), ptr(srt),
          ptr(payload), ptr(first), ptr(mrow), ptr(mlag), ptr(mw),
          stream_ptr())
-    return TilePlan(jt, M, tptr, mrow, mlag, mw)
+    tp = TilePlan(jt, M, P, gstart.contiguous(), tptr, mrow, mlag, mw)
+    if ds:
+        # the same fill with the diffuse weights (mrow and mlag are
+        # written again with the same values)
+        tp.mdw = torch.empty((B, M, jt), dtype=torch.float64, device=dev)
+        call("spr_plan_tile_fill", P, jt, B, Vd, M, ptr(q["row_ptr"]),
+             ptr(q["rowid"]), ptr(q["lag"]), ptr(q["dscal"]), ptr(srt),
+             ptr(payload), ptr(first), ptr(mrow), ptr(mlag), ptr(tp.mdw),
+             stream_ptr())
+        has = q["gend"] > q["gstart"]
+        tp.gdiff = torch.where(has, q["gend"] - 1,
+                               torch.full_like(q["gend"], -1)).contiguous()
+    return tp
+
+
+def block_order(patch_points, patch_to_wall_ids, block=(4, 4)):
+    """Destination order in which every run of block[0]*block[1] slots is
+    one block of neighbouring patches of one wall (the patches of a wall
+    form a regular grid; blocks at a wall's edge are padded with empty
+    slots, -1).  For tile_plan(..., order=...)."""
+    pts = np.asarray(patch_points, float)
+    wall = np.asarray(patch_to_wall_ids)
+    ctr = pts.mean(axis=1)
+    a, b = int(block[0]), int(block[1])
+    out = []
+    for w in np.unique(wall):
+        idx = np.nonzero(wall == w)[0]
+        c = ctr[idx]
+        ext = c.max(0) - c.min(0)
+        ax = np.argsort(-ext, kind="stable")[:2]
+        # grid coordinates along the wall's two in-plane axes
+        size = np.abs(pts[idx, 2] - pts[idx, 0]).max(0)
+        size = np.where(size > 0, size, 1.0)
+        u = np.rint((c[:, ax[0]] - c[:, ax[0]].min()) / size[ax[0]])
+        v = np.rint((c[:, ax[1]] - c[:, ax[1]].min()) / size[ax[1]])
+        u, v = u.astype(np.int64), v.astype(np.int64)
+        bu, bv = u // a, v // b
+        for key in sorted(set(zip(bu.tolist(), bv.tolist()))):
+            mask = (bu == key[0]) & (bv == key[1])
+            sel = idx[mask][np.lexsort((v[mask], u[mask]))]
+            out.extend(sel.tolist())
+            out.extend([-1] * (a * b - len(sel)))
+    return np.array(out, np.int64)
 
 
 def diffuse_specular_tables(scattering, n_directions, n_bins, n_rows):
@@ -566,6 +709,7 @@ def diffuse_specular_plan(vis, patches_center, patches_area, form_factors,
     plan = GroupedPlan(P, D, B, row_ptr, grp_ptr, grp_start, grp_key,
                        rowid[:Vd], lag[:Vd], scal[:, :Vd].contiguous(), rtab,
                        dscal[:, :Vd].contiguous())
+    plan.entry_key = key[:Vd]     # specular direction of each entry (D: none)
     if keep_adjoint:
         # what the adjoint needs besides the plan: entry keys (specular
         # direction, D for none), BRDF rows, pair factors, the tables
@@ -687,6 +831,13 @@ class ExchangeState:
     def gather(self, plan):
         """Grouped plan, pass 1 of an order (per-group sums)."""
         P, D, B, T = self.shape
+        if plan.diffuse_specular and plan.tiles is not None:
+            tp = plan.tiles
+            call("spr_exchange_gather_tiled_ds", tp.P, tp.JT, B, T, tp.M,
+                 ptr(tp.tptr), ptr(tp.gstart), ptr(tp.gdiff), ptr(tp.mrow),
+                 ptr(tp.mlag), ptr(tp.mw), ptr(tp.mdw), ptr(self.prev),
+                 ptr(self._gsum(plan)), stream_ptr())
+            return
         if plan.diffuse_specular:
             call("spr_exchange_gather_ds", P, B, T, plan.Vd,
                  ptr(plan.row_ptr), ptr(plan.grp_ptr), ptr(plan.rowid),
@@ -695,8 +846,8 @@ class ExchangeState:
             return
         if plan.tiles is not None:
             tp = plan.tiles
-            call("spr_exchange_gather_tiled", P, tp.JT, B, T, tp.M,
-                 ptr(tp.tptr), ptr(plan.grp_ptr), ptr(tp.mrow),
+            call("spr_exchange_gather_tiled", tp.P, tp.JT, B, T, tp.M,
+                 ptr(tp.tptr), ptr(tp.gstart), ptr(tp.mrow),
                  ptr(tp.mlag), ptr(tp.mw), ptr(self.prev),
                  ptr(self._gsum(plan)), stream_ptr())
             return

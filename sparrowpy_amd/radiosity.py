@@ -650,6 +650,20 @@ class DirectionalRadiosityFast:
     # TILE_MIN_MERGE entries into one.
     TILE_ROWS = 4
     TILE_MIN_MERGE = 1.5
+    # Tiles as (a, b) blocks of neighbouring patches of one wall instead of
+    # TILE_ROWS consecutive patch ids (a * b must equal TILE_ROWS); None:
+    # consecutive ids.
+    TILE_BLOCK = None
+
+    def _tile_order(self):
+        """Destination order of the tiles (None: consecutive patch ids)."""
+        if self.TILE_BLOCK is None:
+            return None
+        a, bb = self.TILE_BLOCK
+        if a * bb != self.TILE_ROWS:
+            raise ValueError("TILE_BLOCK must hold TILE_ROWS patches")
+        return ops.block_order(self._patches_points,
+                               self._patch_to_wall_ids, self.TILE_BLOCK)
 
     def _exchange_plan(self, speed_of_sound, time_resolution, kind=None):
         """Plan of the exchange (cached per speed of sound and bin width).
@@ -660,7 +674,7 @@ class DirectionalRadiosityFast:
         plus one specular entry, else "generic").
         """
         key = (float(speed_of_sound), float(time_resolution), kind,
-               self.TILE_ROWS, self.TILE_MIN_MERGE)
+               self.TILE_ROWS, self.TILE_MIN_MERGE, self.TILE_BLOCK)
         if getattr(self, "_plan", None) is not None and self._plan[0] == key:
             return self._plan[1]
         b = self._bake
@@ -693,7 +707,8 @@ class DirectionalRadiosityFast:
                 plan = None
             if plan is not None and self.TILE_ROWS:
                 plan.tiles = ops.tile_plan(plan, self.TILE_ROWS,
-                                           self.TILE_MIN_MERGE)
+                                           self.TILE_MIN_MERGE,
+                                           self._tile_order())
         if plan is None and kind in (None, "ds"):
             try:
                 plan = ops.diffuse_specular_plan(*args)
@@ -702,6 +717,11 @@ class DirectionalRadiosityFast:
                     raise
             if plan is None and kind == "ds":
                 raise ValueError("BRDF rows are not diffuse + specular")
+            if plan is not None and self.TILE_ROWS == 16:
+                # 16-row tiles, the size the diffuse + specular tiled
+                # gather supports
+                plan.tiles = ops.tile_plan(plan, 16, self.TILE_MIN_MERGE,
+                                           self._tile_order())
         if plan is None:
             plan = ops.ExchangePlan.from_geometry(*args)
         self._plan = (key, plan)

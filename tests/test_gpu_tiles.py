@@ -45,7 +45,7 @@ def _room(name, per_wall=True, seed=3):
     return r
 
 
-@pytest.mark.parametrize("jt", [4, 8])
+@pytest.mark.parametrize("jt", [4, 8, 16])
 @pytest.mark.parametrize("name", list(ROOMS))
 def test_tile_plan_contents(ops, name, jt):
     r = _room(name)
@@ -100,7 +100,7 @@ def _random_state(ops, P, D, B, T, seed):
     return st
 
 
-@pytest.mark.parametrize("jt", [4, 8])
+@pytest.mark.parametrize("jt", [4, 8, 16])
 @pytest.mark.parametrize("per_wall", [True, False])
 @pytest.mark.parametrize("name", list(ROOMS))
 def test_tiled_gather_bitexact(ops, name, per_wall, jt):
@@ -121,7 +121,7 @@ def test_tiled_gather_bitexact(ops, name, per_wall, jt):
     assert torch.equal(st.gsum[:gp.n_groups], ref)
 
 
-@pytest.mark.parametrize("jt", [4, 8])
+@pytest.mark.parametrize("jt", [4, 8, 16])
 def test_tiled_gather_bitexact_wide_delays(ops, jt):
     """At 48 kHz delays exceed 256 bins, so a row's first bins fall inside
     the workgroups' bin ranges, behind lanes whose bins precede the row
@@ -146,6 +146,57 @@ def test_tiled_gather_bitexact_wide_delays(ops, jt):
     assert torch.equal(st.gsum[:gp.n_groups], ref)
 
 
+@pytest.mark.parametrize("jt,block", [(16, (4, 4)), (8, (2, 4)),
+                                      (4, (2, 2))])
+def test_block_tiles_bitexact(ops, jt, block):
+    """Tiles as blocks of neighbouring patches (rows permuted, walls'
+    edges padded with empty slots): the same group sums, bit for bit, in
+    the plan's own group order (each slot stores at its destination's
+    groups)."""
+    import torch
+    r = _room("3x2.5x2")
+    gp, _ = _grouped_and_generic(ops, r, 343., 1 / 48000)
+    order = ops.block_order(r.patches_points, r.patch_to_wall_ids, block)
+    assert (order < 0).any()          # some blocks are padded
+    tp = ops.tile_plan(gp, jt, min_merge=0, order=order)
+    assert tp.P == len(order) + (-len(order)) % jt
+    P, D, B, T = gp.P, gp.D, gp.B, 1500
+    st = _random_state(ops, P, D, B, T, seed=5)
+    st.gather(gp)
+    ref = st.gsum[:gp.n_groups].clone()
+    st.gsum.fill_(-1.0)
+    gp.tiles = tp
+    st.gather(gp)
+    torch.cuda.synchronize()
+    assert torch.equal(st.gsum[:gp.n_groups], ref)
+
+
+@pytest.mark.parametrize("block", [None, (4, 4)])
+@pytest.mark.parametrize("per_wall", [True, False])
+def test_ds_tiles_bitexact(ops, per_wall, block):
+    """Diffuse + specular plan on 16-row tiles (merged by each entry's
+    specular direction, one wave per row, LDS-staged): the specular group
+    sums and the rows' diffuse sums equal ds_gather_kernel's bit for bit,
+    at 48 kHz (delays of hundreds of bins) and with a bin count that is
+    not a multiple of the block."""
+    import torch
+    from test_gpu_parity import _ds_plan
+    r = _room("3x2.5x2", per_wall=per_wall)
+    gp = _ds_plan(ops, r, 343., 1 / 48000)
+    order = None if block is None else ops.block_order(
+        r.patches_points, r.patch_to_wall_ids, block)
+    tp = ops.tile_plan(gp, 16, min_merge=0, order=order)
+    P, D, B, T = gp.P, gp.D, gp.B, 1500
+    st = _random_state(ops, P, D, B, T, seed=7)
+    st.gather(gp)
+    ref = st.gsum[:gp.n_groups].clone()
+    st.gsum.fill_(-1.0)
+    gp.tiles = tp
+    st.gather(gp)
+    torch.cuda.synchronize()
+    assert torch.equal(st.gsum[:gp.n_groups], ref)
+
+
 def _gsum_cpu(gp, prev, T):
     """gsum[g, b, t] = sum_{e in g} scal[b, e] prev[rowid[e], b, t - lag[e]]
     in float64."""
@@ -162,7 +213,7 @@ def _gsum_cpu(gp, prev, T):
     return out
 
 
-@pytest.mark.parametrize("jt", [4, 8])
+@pytest.mark.parametrize("jt", [4, 8, 16])
 def test_tiled_exchange_matches_oracle(ops, jt):
     """Whole exchange on tiles: equal to the untiled grouped plan and
     within the exchange tolerance of the oracle's FP64 loop."""
