@@ -82,19 +82,29 @@ def _generic_rows(plan, D):
     return row_ptr, src, lag, dirs, coef
 
 
-def cpu_baseline(plan, D, T, seconds, seed=0):
+def _host_cores():
+    """(threads this process may use, logical CPUs of the host)."""
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        usable = os.cpu_count() or 1
+    return usable, os.cpu_count() or 1
+
+
+def cpu_baseline(plan, D, T, seconds, nthreads, seed=0):
     """Time the FP64 oracle's exchange order on a bounded sample.
 
     Sample: a seeded subset of destination patches with their full source
-    lists, band 0 only, the histogram truncated to ``T_s`` bins, all host
-    threads the process may use.  The oracle performs the reference's
-    per-pair D-wide update.  Returns (updates/s, threads, T_s, n_dest).
+    lists, band 0 only, the histogram truncated to ``T_s`` bins, sized so
+    the timed call takes about ``seconds``.  The oracle performs the
+    reference's per-pair D-wide update (RadiosityFast.py:1133-1143);
+    ``nthreads`` = 1 is the serial loop of the reference's numba.njit
+    exchange (RadiosityFast.py:1396), more threads split the destinations.
+    Returns (updates/s, T_s, n_dest).
     """
     import oracle as orc
     orc.build()
     lib = orc._load()
-    nthreads = int(os.environ.get("OMP_NUM_THREADS",
-                                  min(os.cpu_count() or 1, 16)))
     row_ptr, src, lag, dirs, coef_all = _generic_rows(plan, D)
     P = plan.P
     T_s = min(T, 2400)
@@ -119,31 +129,113 @@ def cpu_baseline(plan, D, T, seconds, seed=0):
         return upd, time.perf_counter() - t0
 
     # calibrate on a few destinations, then size the sample to `seconds`
-    upd, dt = run(order[:8])
-    per_dest = max(dt / 8, 1e-6)
-    n_dest = int(min(P, max(8, seconds / per_dest)))
+    n0 = max(2, min(8, nthreads))
+    upd, dt = run(order[:n0])
+    per_dest = max(dt / n0, 1e-6) / max(1, min(nthreads, n0))
+    n_dest = int(min(P, max(n0, seconds / per_dest)))
     upd, dt = run(order[:n_dest])
-    return upd / dt, nthreads, T_s, n_dest
+    return upd / dt, T_s, n_dest
+
+
+def _plan_kind(plan):
+    from sparrowpy_amd import ops
+    if not isinstance(plan, ops.GroupedPlan):
+        return "generic"
+    return "ds" if plan.diffuse_specular else "grouped"
+
+
+def byte_models(plan, P, D, B, T, U, n_pair_per_order):
+    """HBM bytes per launch of the order's kernels.
+
+    ``min``: compulsory traffic, what a kernel must move at least: every
+    source row (patch, outgoing direction) the plan reads is read once
+    (B*T floats each), every output written once, every plan word read
+    once.  ``reference``: the accounting of one update per gathered float
+    (4 bytes per update, the F~ slice and delay/direction per directed
+    pair, the order's histogram written, the total read and written);
+    cache reuse lets a kernel beat it, so it is no roofline.
+    Returns {kernel: {"min": bytes, "reference": bytes}}, with the
+    group-sum round trip of the two-pass plans on its own line.
+    """
+    import torch
+    kind = _plan_kind(plan)
+    Vd = plan.Vd
+    n_state = P * D * B * T
+    total_rw = 20 * n_state * n_pair_per_order  # prev + hi/lo read + write
+    out = {}
+    if kind == "generic":
+        rows = int(torch.unique(plan.src.to(torch.int64) * D +
+                                plan.dir.to(torch.int64)).numel())
+        meta = Vd * (12 + 8 * D * B) + 8 * (P + 1)
+        out["exchange_step_kernel"] = {
+            "min": 4 * B * T * rows + meta + 4 * n_state + 16 * n_state,
+            "reference": 4 * U + Vd * (4 * D * B + 8) + 12 * n_state}
+        return out
+    rows = int(torch.unique(plan.rowid).numel())
+    G = plan.n_groups
+    gsum = 4 * G * B * T
+    tiles = getattr(plan, "tiles", None)
+    n_w = 2 if kind == "ds" else 1
+    if tiles is not None:
+        JT, M = tiles.JT, tiles.M
+        meta = M * (4 + 4 * JT + 8 * JT * B * n_w) + 8 * (
+            tiles.tptr.numel() + n_w * tiles.P)
+        gname = "tile_lds_gather_kernel" if JT == 16 else \
+            "tile_gather_kernel"
+    else:
+        meta = Vd * (8 + 8 * B * n_w) + 8 * (2 * P + 2)
+        gname = "ds_gather_kernel" if kind == "ds" else "group_gather_kernel"
+    out[gname] = {"min": 4 * B * T * rows + meta + gsum,
+                  "reference": 4 * U + Vd * (8 + 8 * B * n_w) + gsum,
+                  "group_sums": gsum}
+    sname = "ds_spread_kernel" if kind == "ds" else "group_spread_kernel"
+    spread_meta = 4 * G + 8 * (P + 1) + (
+        0 if kind == "ds" else plan.rtab.numel() * 8)
+    out[sname] = {"min": gsum + 4 * n_state + total_rw + spread_meta,
+                  "reference": gsum + 4 * n_state + total_rw,
+                  "group_sums": gsum}
+    return out
+
+
+TRAFFIC_PROFILE = os.path.join(os.path.dirname(os.path.abspath(__file__)),
+                               "profiles", "traffic_street_canyon.json")
+KERNEL_SOURCES = ["exchange.hip", "tiles.hip", "spr_exchange.hpp",
+                  "spr_util.hpp", "spr_common.hpp"]
+
+
+def kernel_sources_sha():
+    """sha256 over the exchange kernels' sources: a counter profile is only
+    used for the kernels it was measured on."""
+    import hashlib
+    h = hashlib.sha256()
+    csrc = os.path.join(os.path.dirname(os.path.abspath(__file__)),
+                        "sparrowpy_amd", "csrc")
+    for name in KERNEL_SOURCES:
+        with open(os.path.join(csrc, name), "rb") as f:
+            h.update(name.encode() + b"\0" + f.read())
+    return h.hexdigest()[:16]
 
 
 def measured_traffic(kernel, config):
     """HBM bytes per launch of ``kernel`` from the committed rocprofv3
     counter summary of this workload (FETCH_SIZE and WRITE_SIZE passes,
-    scaled by a copy of known size), or None if it does not cover it."""
-    path = os.path.join(os.path.dirname(os.path.abspath(__file__)),
-                        "profiles", "traffic_street_canyon.json")
+    scaled by a copy of known size), or (None, reason) when the profile
+    covers another workload or other kernel sources."""
     try:
-        with open(path) as f:
+        with open(TRAFFIC_PROFILE) as f:
             prof = json.load(f)
     except (OSError, ValueError):
-        return None
+        return None, "no counter profile"
     if any(prof["workload"].get(k) != config.get(k)
            for k in prof["workload"]):
-        return None
+        return None, "counter profile of another workload"
+    if prof.get("kernel_sources_sha") != kernel_sources_sha():
+        return None, ("counter profile measured on other kernel sources "
+                      "(stale; regenerate with bench.py --traffic-from)")
     for name, v in prof["kernels"].items():
         if name.split("<")[0] == kernel:
-            return v["bytes"]
-    return None
+            return v["bytes"], None
+    return None, f"{kernel} not in the counter profile"
 
 
 def adjoint_timing(log):
@@ -226,9 +318,12 @@ def bench_inverse(args, log):
                                      torch.tensor(ts, **kw), sv)
     target = adj.forward(A, C).to(torch.float64)
     adj.release()
+    # start: every patch's absorption 0.05 too high and scattering 0.1 too
+    # low, plus per-patch noise (the per-band means are what 4 receivers
+    # identify; the per-patch values are not determined by them)
     rng = np.random.default_rng(11)
-    a0 = np.clip(ta + rng.normal(0, .05, ta.shape), .01, .99)
-    s0 = np.clip(ts + rng.normal(0, .1, ts.shape), .01, .99)
+    a0 = np.clip(ta + .05 + rng.normal(0, .03, ta.shape), .01, .99)
+    s0 = np.clip(ts - .1 + rng.normal(0, .05, ts.shape), .01, .99)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     a, s, losses = adjoint.recover_materials(adj, target, a0, s0, sv,
@@ -255,6 +350,11 @@ def bench_inverse(args, log):
                           float(np.abs(a - ta).mean())],
         "abs_err_scattering": [float(np.abs(s0 - ts).mean()),
                                float(np.abs(s - ts).mean())],
+        "band_mean_err_alpha": [np.abs(a0.mean(0) - ta.mean(0)).tolist(),
+                                np.abs(a.mean(0) - ta.mean(0)).tolist()],
+        "band_mean_err_scattering": [
+            np.abs(s0.mean(0) - ts.mean(0)).tolist(),
+            np.abs(s.mean(0) - ts.mean(0)).tolist()],
     }
 
 
@@ -288,21 +388,32 @@ def bench_batched(args, log):
     # the batched run first: its chunk buffers are tens of GiB each, and
     # the per-instance loop leaves the device heap fragmented (each room's
     # plan is placed in the previous room's freed histograms)
-    be = BatchedExchange(items, c, dt, dur, K)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    out = be.run()
-    torch.cuda.synchronize()
-    batched_s = time.perf_counter() - t0
-    log(f"batched {batched_s:.2f} s, classes {be.choice}")
-    be.pool = None
-    torch.cuda.empty_cache()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    ref = run_per_instance(items, c, dt, dur, K)
-    torch.cuda.synchronize()
-    loop_s = time.perf_counter() - t0
-    log(f"per-instance loop {loop_s:.2f} s")
+    batched_s = loop_s = float("nan")
+    out = ref = None
+    if args.batched_part in ("both", "batched"):
+        be = BatchedExchange(items, c, dt, dur, K,
+                             tile_rows=args.batch_tile_rows)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        out = be.run()
+        torch.cuda.synchronize()
+        batched_s = time.perf_counter() - t0
+        log(f"batched {batched_s:.2f} s, classes {be.choice}")
+        be.pool = None
+        torch.cuda.empty_cache()
+    if args.batched_part in ("both", "loop"):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        ref = run_per_instance(items, c, dt, dur, K)
+        torch.cuda.synchronize()
+        loop_s = time.perf_counter() - t0
+        log(f"per-instance loop {loop_s:.2f} s")
+    if out is None or ref is None:
+        U = be.updates if out is not None else None
+        return {"metric": "batched rooms, one part only",
+                "part": args.batched_part, "batched_s": batched_s,
+                "loop_s": loop_s, "updates": U, "n_gpus": 1,
+                "instances": n_inst}
     err = 0.0
     for key, v in ref.items():
         a, b = out[key].double(), v.double()
@@ -337,6 +448,58 @@ def bench_batched(args, log):
     }
 
 
+def _counter_means(root, counter):
+    """{kernel name: mean value of ``counter`` per dispatch} from the
+    rocprofv3 CSV output under ``root`` (``*counter_collection.csv``)."""
+    import csv
+    import glob
+    vals = {}
+    files = glob.glob(os.path.join(root, "**", "*counter_collection.csv"),
+                      recursive=True)
+    if not files:
+        raise FileNotFoundError(f"no counter_collection.csv under {root}")
+    for path in files:
+        with open(path, newline="") as f:
+            for row in csv.DictReader(f):
+                if row.get("Counter_Name") != counter:
+                    continue
+                name = row["Kernel_Name"].split("(")[0].strip()
+                key = (name, row.get("Dispatch_Id", len(vals)))
+                vals[key] = vals.get(key, 0.0) + float(row["Counter_Value"])
+    per = {}
+    for (name, _), v in vals.items():
+        per.setdefault(name, []).append(v)
+    return {k: float(np.mean(v)) for k, v in per.items()}
+
+
+def traffic_from(dirs, workload):
+    """Write the counter profile from four rocprofv3 runs: FETCH_SIZE and
+    WRITE_SIZE passes of the benchmark and of ``--copy-calibration`` (a
+    device copy of known size, which fixes the counters' scale)."""
+    fetch_dir, write_dir, copyf_dir, copyw_dir = dirs
+    copy_name = "__amd_rocclr_copyBuffer"
+    cf = _counter_means(copyf_dir, "FETCH_SIZE")
+    cw = _counter_means(copyw_dir, "WRITE_SIZE")
+    n = 1 << 32
+    fetch_scale = n / (cf[copy_name] * 1024)
+    write_scale = n / (cw[copy_name] * 1024)
+    fe = _counter_means(fetch_dir, "FETCH_SIZE")
+    wr = _counter_means(write_dir, "WRITE_SIZE")
+    kernels = {}
+    for name in sorted(set(fe) & set(wr)):
+        fb = fe[name] * 1024 * fetch_scale
+        wb = wr[name] * 1024 * write_scale
+        kernels[name] = {"fetch_bytes": fb, "write_bytes": wb,
+                         "bytes": fb + wb, "fetch_kib_raw": fe[name],
+                         "write_kib_raw": wr[name]}
+    out = {"workload": workload, "kernel_sources_sha": kernel_sources_sha(),
+           "fetch_scale": fetch_scale, "write_scale": write_scale,
+           "kernels": kernels}
+    with open(TRAFFIC_PROFILE, "w") as f:
+        json.dump(out, f, indent=1)
+    return out
+
+
 def copy_calibration(n_bytes=1 << 32, reps=3):
     """Copy n_bytes device-to-device ``reps`` times (one kernel each)."""
     import torch
@@ -364,21 +527,53 @@ def main(argv=None):
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--rooms", type=int, default=256,
                     help="rooms of --config batched")
+    ap.add_argument("--batched-part", default="both",
+                    choices=["both", "batched", "loop"],
+                    help="--config batched: run both paths, or only one "
+                         "(to profile it alone)")
+    ap.add_argument("--batch-tile-rows", type=int, default=4,
+                    choices=[0, 4, 8, 16],
+                    help="--config batched: destination rows per tile of "
+                         "the batched classes' gathers")
     ap.add_argument("--no-adjoint", action="store_true",
                     help="skip the adjoint/forward timing")
     ap.add_argument("--plan", default="auto",
                     choices=["auto", "ds", "grouped", "generic"],
                     help="exchange plan (auto: grouped when sources share "
                          "BRDF rows)")
-    ap.add_argument("--tile-rows", type=int, default=None, choices=[0, 4, 8],
+    ap.add_argument("--tile-rows", type=int, default=None, choices=[0, 4, 8, 16],
                     help="destination rows per tile of the grouped gather "
                          "(default: the class default; 0: row by row)")
+    ap.add_argument("--tile-block", default=None,
+                    help="tiles as AxB blocks of neighbouring patches "
+                         "(e.g. 4x4 with --tile-rows 16); default: the "
+                         "class default")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a device copy of known size (to scale "
                          "the FETCH_SIZE/WRITE_SIZE counters)")
+    ap.add_argument("--traffic-from", nargs=4, default=None,
+                    metavar=("FETCH_DIR", "WRITE_DIR", "COPY_FETCH_DIR",
+                             "COPY_WRITE_DIR"),
+                    help="write profiles/traffic_street_canyon.json from "
+                         "rocprofv3 counter runs of this benchmark and of "
+                         "--copy-calibration, then exit")
     args = ap.parse_args(argv)
     if args.copy_calibration:
         return copy_calibration()
+    if args.traffic_from:
+        from sparrowpy_amd import geometry
+        sc = _scene(args.config)
+        wp = np.array([w.pts for w in sc.walls])
+        wn = np.array([w.normal for w in sc.walls])
+        n_patches = int(geometry.process_patches(wp, wn,
+                                                 sc.patch_size)[2])
+        workload = {"patches": n_patches, "directions":
+                    int(sc.directions.cartesian.shape[0]),
+                    "bands": len(sc.frequencies), "bins": sc.n_samples}
+        prof = traffic_from(args.traffic_from, workload)
+        print(json.dumps({k: v["bytes"] for k, v in
+                          prof["kernels"].items()}, indent=1))
+        return 0
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not "
               "supported (the exchange shards by band with no collectives; "
@@ -414,6 +609,9 @@ def main(argv=None):
     r.set_air_attenuation(FrequencyData(sc.air_attenuation, sc.frequencies))
     if args.tile_rows is not None:
         r.TILE_ROWS = args.tile_rows
+    if args.tile_block is not None:
+        r.TILE_BLOCK = (None if args.tile_block == "none" else
+                        tuple(int(x) for x in args.tile_block.split("x")))
     t["host_setup_s"] = tic() - t0
     t0 = tic()
     r.bake_geometry()
@@ -488,39 +686,28 @@ def main(argv=None):
 
     value = U * args.steps / wall
     n_state = P * D * B * T
-    # algorithmic bytes per order (reference accounting): gathered source
-    # bins, the F~ slice and delay/direction of each directed pair, one write
-    # of the order's histogram and a read + write of the running total
-    order_bytes = 4 * U + Vd * (4 * D * B + 8) + 12 * n_state
+    models = byte_models(plan, P, D, B, T, U, n_pair / args.steps)
     kernels = {}
     if two_pass:
-        G = plan.n_groups
-        # gather: gathered bins, per-pair row/delay + per-band weight(s),
-        # one write of the per-group sums
-        n_w = 2 if kind == "ds" else 1
-        gather_bytes = 4 * U + Vd * (8 + 8 * B * n_w) + 4 * G * B * T
-        # spread: read the group sums, write cur; every other order also
-        # reads the previous order and reads + writes hi and lo (20 bytes
-        # per element)
-        spread_bytes = (4 * G * B * T + 4 * n_state +
-                        20 * n_state * n_pair / args.steps)
-        gather_name = ("tile_gather_kernel" if tiles is not None else
-                       "ds_gather_kernel" if kind == "ds" else
-                       "group_gather_kernel")
-        spread_name = ("ds_spread_kernel" if kind == "ds" else
-                       "group_spread_kernel")
+        gather_name, spread_name = list(models)
         kernels = {
-            gather_name: {
-                "ms": gather_s * 1e3, "algorithmic_bytes": gather_bytes,
-                "GB/s": gather_bytes / gather_s / 1e9},
-            spread_name: {
-                "ms": spread_s * 1e3, "algorithmic_bytes": spread_bytes,
-                "GB/s": spread_bytes / spread_s / 1e9},
+            gather_name: dict(ms=gather_s * 1e3, **models[gather_name]),
+            spread_name: dict(ms=spread_s * 1e3, **models[spread_name]),
         }
-        dom, dom_s, dom_bytes = gather_name, gather_s, gather_bytes
+        dom, dom_s = gather_name, gather_s
     else:
-        dom, dom_s, dom_bytes = "exchange_step_kernel", step_s, order_bytes
-    achieved_gbs = dom_bytes / dom_s / 1e9
+        dom, dom_s = "exchange_step_kernel", step_s
+        kernels = {dom: dict(ms=step_s * 1e3, **models[dom])}
+    for k in kernels.values():
+        k["GB/s_min"] = k["min"] / (k["ms"] / 1e3) / 1e9
+        k["GB/s_reference"] = k["reference"] / (k["ms"] / 1e3) / 1e9
+    dom_min = models[dom]["min"]
+    achieved_gbs = dom_min / dom_s / 1e9
+    # the whole order: compulsory bytes without the group sums (which a
+    # one-pass exchange would not move), the group-sum round trip apart
+    gsum_rt = 2 * models[dom].get("group_sums", 0)
+    order_min = sum(m["min"] for m in models.values()) - gsum_rt
+    order_ref = 4 * U + Vd * (4 * D * B + 8) + 12 * n_state
     flops = 2 * D * U
     line = {
         "metric": "patch-pair·bin updates/s at 5k patches",
@@ -551,38 +738,67 @@ def main(argv=None):
             "traffic": None,
             "traffic_source": None,
             "kernel": dom, "kernel_ms": dom_s * 1e3,
-            "algorithmic_bytes": dom_bytes,
+            "algorithmic_bytes": dom_min,
+            "bytes_model": "compulsory: each source row the plan reads "
+                           "once (B*T floats), each output written once, "
+                           "plan metadata once",
+            "reference_equivalent_bytes": models[dom]["reference"],
+            "reference_equivalent_GB/s":
+                models[dom]["reference"] / dom_s / 1e9,
         },
         "order": {
-            "ms": step_s * 1e3, "algorithmic_bytes": order_bytes,
-            "GB/s": order_bytes / step_s / 1e9,
-            "frac_hbm": order_bytes / step_s / 1e9 / HBM_PEAK_GBS,
+            "ms": step_s * 1e3,
+            "min_bytes": order_min,
+            "group_sum_round_trip_bytes": gsum_rt,
+            "GB/s_min": order_min / step_s / 1e9,
+            "frac_hbm_min": order_min / step_s / 1e9 / HBM_PEAK_GBS,
+            "frac_hbm_min_with_group_sums":
+                (order_min + gsum_rt) / step_s / 1e9 / HBM_PEAK_GBS,
+            "reference_equivalent_bytes": order_ref,
+            "reference_equivalent_GB/s": order_ref / step_s / 1e9,
             "reference_flops_tflops": flops / step_s / 1e12,
             "kernels": kernels,
         },
         "setup": t,
     }
-    traffic = measured_traffic(dom, line["config"])
+    traffic, why = measured_traffic(dom, line["config"])
     if traffic is not None:
         line["roofline"]["traffic"] = traffic
         line["roofline"]["traffic_source"] = (
-            "profiles/exchange_street_canyon.md (rocprofv3 FETCH_SIZE x2 + "
-            "WRITE_SIZE x1, per launch)")
+            "profiles/traffic_street_canyon.json (rocprofv3 FETCH_SIZE and "
+            "WRITE_SIZE passes scaled by a copy of known size, per launch)")
         line["roofline"]["traffic_GB/s"] = traffic / dom_s / 1e9
+    else:
+        line["roofline"]["traffic_source"] = why
     if not args.no_adjoint:
         del state, etc, hist, r
         torch.cuda.empty_cache()
         line["adjoint"] = adjoint_timing(log)
     if not args.no_cpu_baseline:
-        log("cpu baseline sample")
-        ups, nth, T_s, n_dest = cpu_baseline(plan, D, T, args.cpu_seconds)
+        usable, logical = _host_cores()
+        nth = int(os.environ.get("OMP_NUM_THREADS", usable))
+        log(f"cpu baseline sample, {nth} threads")
+        ups, T_s, n_dest = cpu_baseline(plan, D, T, args.cpu_seconds, nth)
+        log("cpu baseline sample, serial")
+        ups1, T_s1, n_dest1 = cpu_baseline(plan, D, T,
+                                           args.cpu_seconds / 2, 1, seed=1)
+        host = (f"{_cpu_info()}; {logical} logical CPUs on the host, "
+                f"{usable} usable by this process")
         line["cpu_baseline"] = {
             "value": ups, "unit": "updates/s", "cores": nth, "kind": "port",
             "sample": f"FP64 C oracle (oracle/sparrow_oracle.c "
-                      f"orc_exchange_order, OpenMP over destinations) on "
+                      f"orc_exchange_order: the reference's per-pair D-wide "
+                      f"update, OpenMP over destinations, {nth} threads) on "
                       f"{n_dest} seeded destination patches of the same "
                       f"scene with their full source lists, band 0, "
-                      f"{T_s} bins; host: {_cpu_info()}",
+                      f"{T_s} bins; host: {host}",
+            "serial": {
+                "value": ups1, "unit": "updates/s", "cores": 1,
+                "sample": f"the same loop on one thread (the reference's "
+                          f"numba.njit exchange is serial, "
+                          f"RadiosityFast.py:1396) on {n_dest1} "
+                          f"destinations, band 0, {T_s1} bins"},
+            "host_logical_cpus": logical, "usable_cpus": usable,
         }
     print(json.dumps(line), flush=True)
     return 0
