@@ -391,8 +391,11 @@ def bench_batched(args, log):
     batched_s = loop_s = float("nan")
     out = ref = None
     if args.batched_part in ("both", "batched"):
-        be = BatchedExchange(items, c, dt, dur, K,
-                             tile_rows=args.batch_tile_rows)
+        be = BatchedExchange(
+            items, c, dt, dur, K, tile_rows=args.batch_tile_rows,
+            tile_block=(None if args.batch_tile_block is None else tuple(
+                int(v) for v in args.batch_tile_block.split("x"))),
+            tile_r=args.tile_r if args.tile_r is not None else 16)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         out = be.run()
@@ -535,6 +538,10 @@ def main(argv=None):
                     choices=[0, 4, 8, 16],
                     help="--config batched: destination rows per tile of "
                          "the batched classes' gathers")
+    ap.add_argument("--batch-tile-block", default=None,
+                    help="--config batched: tiles as AxB blocks of each "
+                         "room's patches (e.g. 4x4 with --batch-tile-rows "
+                         "16)")
     ap.add_argument("--no-adjoint", action="store_true",
                     help="skip the adjoint/forward timing")
     ap.add_argument("--plan", default="auto",
@@ -548,6 +555,10 @@ def main(argv=None):
                     help="tiles as AxB blocks of neighbouring patches "
                          "(e.g. 4x4 with --tile-rows 16); default: the "
                          "class default")
+    ap.add_argument("--tile-r", type=int, default=None,
+                    choices=[16, 8, -8, -16],
+                    help="16-row tiles: bins per lane of the gather (-8: 8 "
+                         "with two batches of loads in flight)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a device copy of known size (to scale "
                          "the FETCH_SIZE/WRITE_SIZE counters)")
@@ -609,9 +620,14 @@ def main(argv=None):
     r.set_air_attenuation(FrequencyData(sc.air_attenuation, sc.frequencies))
     if args.tile_rows is not None:
         r.TILE_ROWS = args.tile_rows
+    if args.tile_r is not None:
+        r.TILE_R = args.tile_r
     if args.tile_block is not None:
-        r.TILE_BLOCK = (None if args.tile_block == "none" else
-                        tuple(int(x) for x in args.tile_block.split("x")))
+        blk = args.tile_block.rstrip("u")
+        r.TILE_BLOCK = (None if blk == "none" else
+                        tuple(int(x) for x in blk.split("x")))
+        # a trailing "u": blocks run together without padding
+        r.TILE_PAD = not args.tile_block.endswith("u")
     t["host_setup_s"] = tic() - t0
     t0 = tic()
     r.bake_geometry()

@@ -428,6 +428,20 @@ int spr_exchange_gather_tiled(int64_t P, int64_t JT, int64_t B, int64_t T,
                               const int32_t *mlag, const double *mw,
                               const float *prev, float *gsum, void *stream);
 
+/* spr_exchange_gather_tiled on 16-row tiles with a chosen workgroup shape:
+ * R = 16 bins per lane (1,024-bin blocks, one workgroup per CU) or 8
+ * (512-bin blocks, two workgroups per CU); the same group sums, bit for
+ * bit.  For autotuning.
+ * Replaces, with a spread, one order of _energy_exchange,
+ * RadiosityFast.py:1121-1144. */
+int spr_exchange_gather_tiled_shape(int64_t P, int64_t JT, int64_t R,
+                                    int64_t B, int64_t T, int64_t M,
+                                    const int64_t *tptr,
+                                    const int64_t *grp_ptr,
+                                    const int32_t *mrow, const int32_t *mlag,
+                                    const double *mw, const float *prev,
+                                    float *gsum, void *stream);
+
 /* Gather pass of the diffuse + specular plan on 16-row tiles (merged by
  * each entry's specular direction): gsum bit-identical to
  * spr_exchange_gather_ds, for use with spr_exchange_spread_ds.  mw and

@@ -109,6 +109,9 @@ SYMBOLS = {
                                   _vp]),
     "spr_exchange_gather_tiled": (_int, [_i64, _i64, _i64, _i64, _i64, _vp,
                                          _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "spr_exchange_gather_tiled_shape": (_int, [_i64, _i64, _i64, _i64,
+                                               _i64, _i64, _vp, _vp, _vp,
+                                               _vp, _vp, _vp, _vp, _vp]),
     "spr_exchange_gather_tiled_ds": (_int, [_i64, _i64, _i64, _i64, _i64,
                                             _vp, _vp, _vp, _vp, _vp, _vp,
                                             _vp, _vp, _vp, _vp]),

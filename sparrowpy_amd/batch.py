@@ -103,9 +103,11 @@ class BatchedExchange:
 
     def __init__(self, items, speed_of_sound, time_resolution, duration,
                  max_order, autotune=True, memory_budget=None, graphs=True,
-                 tile_rows=4):
+                 tile_rows=4, tile_block=None, tile_r=16):
         self.items = list(items)
         self.tile_rows = int(tile_rows)
+        self.tile_block = tile_block   # e.g. (4, 4) with tile_rows 16
+        self.tile_r = int(tile_r)
         self.c, self.dt = float(speed_of_sound), float(time_resolution)
         self.T = int(duration / time_resolution)
         self.K = int(max_order)
@@ -228,8 +230,22 @@ class BatchedExchange:
         plan = _concat_plans([plans[x] for x in xs], offsets[:-1])
         if self.tile_rows and isinstance(plan, ops.GroupedPlan) and (
                 not plan.diffuse_specular or self.tile_rows == 16):
-            # diffuse + specular plans tile at 16 rows only
-            plan.tiles = ops.tile_plan(plan, self.tile_rows)
+            # diffuse + specular plans tile at 16 rows only; with a tile
+            # block, each room's patches in blocks of that shape, the rooms
+            # one after the other
+            order = None
+            if self.tile_block is not None:
+                parts = []
+                for x, off in zip(xs, offsets[:-1]):
+                    r = self.items[x].radiosity
+                    o = ops.block_order(r._patches_points,
+                                        r._patch_to_wall_ids,
+                                        self.tile_block)
+                    parts.append(np.where(o >= 0, o + off, -1))
+                order = np.concatenate(parts)
+            plan.tiles = ops.tile_plan(plan, self.tile_rows, order=order)
+            if plan.tiles is not None:
+                plan.tiles.R = self.tile_r
         e0d, d0 = [], []
         for x, s in chunk:
             r = self.items[x].radiosity

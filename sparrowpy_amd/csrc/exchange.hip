@@ -534,7 +534,10 @@ __global__ __launch_bounds__(256) void group_spread_kernel(
   const int tb = (int)(tile % n_tblk);
   const int64_t j = (tile / n_tblk) % P;
   const int b = (int)(tile / (n_tblk * P));
-  const int wave = (int)threadIdx.x >> 6, lane = (int)threadIdx.x & 63;
+  // the wave index as a scalar (readfirstlane): the BRDF-row reads below
+  // then stay scalar loads shared by the wave, not per-lane loads
+  const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
+  const int lane = (int)threadIdx.x & 63;
   const int wd = wave % Sh::NWD, wt = wave / Sh::NWD;
   const int d0 = wd * DW;  // first direction of this wave
   const int tl = tb * Sh::TILE + wt * (64 * R) + lane;  // bin of r = 0

@@ -343,7 +343,7 @@ int launch_tile_shape(int64_t P, int64_t B, int64_t T, int64_t M,
   return SPR_OK;
 }
 
-template <int R, int JT, int NB, int SMAX, bool DS>
+template <int R, int JT, int NB, int SMAX, bool DS, int DEPTH, bool PIPE>
 int launch_tile_lds(int64_t P, int64_t B, int64_t T, int64_t M,
                     const int64_t *tptr, const int64_t *grp_ptr,
                     const int64_t *gdiff, const int32_t *mrow,
@@ -361,9 +361,9 @@ int launch_tile_gather(int64_t P, int JT, int64_t B, int64_t T, int64_t M,
   if (JT == 8)
     return launch_tile_shape<4, 1, 8, 4, 64>(P, B, T, M, tptr, grp_ptr,
                                              mrow, mlag, mw, prev, gsum, s);
-  return launch_tile_lds<16, 16, 16, 64, false>(P, B, T, M, tptr, grp_ptr,
-                                                nullptr, mrow, mlag, mw,
-                                                nullptr, prev, gsum, s);
+  return launch_tile_lds<16, 16, 16, 64, false, 1, false>(
+      P, B, T, M, tptr, grp_ptr, nullptr, mrow, mlag, mw, nullptr, prev, gsum,
+      s);
 }
 
 // ---------------------------------------------------------------------
@@ -431,7 +431,7 @@ __device__ __forceinline__ SlotMeta<JT> load_slot_meta(
   return t;
 }
 
-template <int R, int JT, int NB, int SMAX, bool DS>
+template <int R, int JT, int NB, int SMAX, bool DS, int DEPTH, bool PIPE>
 __global__ __launch_bounds__(64 * JT) void tile_lds_gather_kernel(
     int64_t P, int B, int T, int n_tblk, int64_t nblocks, int64_t M,
     const int64_t *__restrict__ tptr, const int64_t *__restrict__ grp_ptr,
@@ -444,6 +444,8 @@ __global__ __launch_bounds__(64 * JT) void tile_lds_gather_kernel(
   constexpr int NX = W / 64;       // window floats per lane
   constexpr int NE = NB / JT;      // entries each wave stages per batch
   static_assert(SMAX % 64 == 0 && NB % JT == 0 && 64 % NB == 0, "shape");
+  static_assert(DEPTH == 1 || DEPTH == 2, "one or two batches in flight");
+  static_assert(64 / NB >= DEPTH, "staging stays within the next block");
   __shared__ float lds[2][NB][W];
   int64_t tile;
   int b, tb;
@@ -452,7 +454,10 @@ __global__ __launch_bounds__(64 * JT) void tile_lds_gather_kernel(
   const int64_t m_begin = tptr[tile], m_end = tptr[tile + 1];
   if (m_end <= m_begin) return;  // uniform: before any barrier
   const int t0 = tb * TB;
-  const int k = (int)(threadIdx.x >> 6);   // slot = wave
+  // slot = wave; readfirstlane makes it (and everything derived from it:
+  // the slot's row, its group counter, its branches) a scalar value, so
+  // the per-entry branches are scalar branches, not exec-mask juggling
+  const int k = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   const int lane = (int)(threadIdx.x & 63);
   const int64_t j = tile * JT + k;
   const bool active = j < P;
@@ -462,9 +467,12 @@ __global__ __launch_bounds__(64 * JT) void tile_lds_gather_kernel(
   const uint32_t row_bytes = (uint32_t)T * 4u;
   const int64_t bstride = (int64_t)B * T;
 
-  // batch staging: wave k loads entries k, k + JT, ... of the batch
-  float st[NE][NX];
-  auto stage_load = [&](const SlotMeta<JT> &m, int64_t mb, int64_t blk0) {
+  // batch staging: wave k loads entries k, k + JT, ... of a batch into
+  // registers st[slot]; DEPTH register sets, so with DEPTH = 2 the loads of
+  // batch q+2 are issued while batch q+1's are still in flight
+  float st[DEPTH][NE][NX];
+  auto stage_load = [&](int slot, const SlotMeta<JT> &m, int64_t mb,
+                        int64_t blk0) {
 #pragma unroll
     for (int e = 0; e < NE; ++e) {
       const int64_t mm = mb + k + e * JT;
@@ -480,16 +488,17 @@ __global__ __launch_bounds__(64 * JT) void tile_lds_gather_kernel(
       const int s0 = t0 - mx + lane;
 #pragma unroll
       for (int x = 0; x < NX; ++x)
-        st[e][x] = 64 * x < need ? buffer_load(rs, bin_offset(s0 + 64 * x))
-                                 : 0.0f;
+        st[slot][e][x] = 64 * x < need
+                             ? buffer_load(rs, bin_offset(s0 + 64 * x))
+                             : 0.0f;
     }
   };
-  auto stage_write = [&](int buf) {
+  auto stage_write = [&](int slot, int buf) {
 #pragma unroll
     for (int e = 0; e < NE; ++e)
 #pragma unroll
       for (int x = 0; x < NX; ++x)
-        lds[buf][k + e * JT][lane + 64 * x] = st[e][x];
+        lds[buf][k + e * JT][lane + 64 * x] = st[slot][e][x];
   };
 
   double S[R], Sd[DS ? R : 1];
@@ -549,34 +558,135 @@ __global__ __launch_bounds__(64 * JT) void tile_lds_gather_kernel(
     }
   };
 
+  // The same sums, software-pipelined: the wave walks only the entries of
+  // the batch where its slot is present or ends a group (set bits of a
+  // ballot over the metadata lanes), and issues the next present entry's
+  // LDS reads into the other register set before summing the current one.
+  auto read_entry = [&](const SlotMeta<JT> &m, int idx, int e, int buf,
+                        float (&v)[R]) {
+    const int32_t l = __builtin_amdgcn_readlane(m.lag, idx) & kLagMask;
+    const int32_t mx = __builtin_amdgcn_readlane(m.maxlag, idx);
+    const int off = mx - l;
+    if (off <= SMAX) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[r] = lds[buf][e][off + lane + 64 * r];
+    } else {
+      const int32_t row = __builtin_amdgcn_readlane(m.row, idx);
+      const __amdgpu_buffer_rsrc_t rs = make_rsrc(
+          prev + (int64_t)row * bstride + (int64_t)b * T, row_bytes);
+#pragma unroll
+      for (int r = 0; r < R; ++r)
+        v[r] = buffer_load(rs, bin_offset(t0 + lane + 64 * r - l));
+    }
+  };
+  auto use_entry = [&](const SlotMeta<JT> &m, int idx, bool present,
+                       const float (&v)[R]) {
+    const int32_t lw = __builtin_amdgcn_readlane(m.lag, idx);
+    if (present) {
+      const double wk = readlane_f64(m.w, idx);
+      if (DS) {
+        const double dk = readlane_f64(m.dw, idx);
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          const double x = (double)v[r];
+          S[r] = fma(wk, x, S[r]);
+          Sd[DS ? r : 0] = fma(dk, x, Sd[DS ? r : 0]);
+        }
+      } else {
+#pragma unroll
+        for (int r = 0; r < R; ++r) S[r] = fma(wk, (double)v[r], S[r]);
+      }
+    }
+    if (lw & kLastInGroup) {
+      const __amdgpu_buffer_rsrc_t os =
+          make_rsrc(gsum + (g * B + b) * (int64_t)T, row_bytes);
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        buffer_store((float)S[r], os, (uint32_t)(t0 + lane + 64 * r) * 4u);
+        S[r] = 0.0;
+      }
+      g += 1;
+    }
+  };
+  auto compute_pipe = [&](const SlotMeta<JT> &m, int64_t mb, int64_t blk0,
+                          int buf) {
+    if (!active) return;  // uniform per wave
+    const int i0 = (int)(mb - blk0);
+    const int n = (int)min((int64_t)NB, m_end - mb);
+    const uint64_t window = (n >= 64 ? ~0ull : ((1ull << n) - 1ull)) << i0;
+    const int32_t lw = m.lag;  // this lane's entry of the block
+    const uint64_t pres =
+        __ballot((lw & kLagMask) != kLagMask) & window;
+    uint64_t todo = (pres | __ballot((lw & kLastInGroup) != 0)) & window;
+    if (!todo) return;
+    float va[R], vb[R];
+    int idx = __builtin_ctzll(todo);
+    if ((pres >> idx) & 1) read_entry(m, idx, idx - i0, buf, va);
+    while (true) {
+      todo &= todo - 1;
+      int nxt_idx = todo ? __builtin_ctzll(todo) : -1;
+      if (nxt_idx >= 0 && ((pres >> nxt_idx) & 1))
+        read_entry(m, nxt_idx, nxt_idx - i0, buf, vb);
+      use_entry(m, idx, (pres >> idx) & 1, va);
+      if (nxt_idx < 0) break;
+      idx = nxt_idx;
+      todo &= todo - 1;
+      nxt_idx = todo ? __builtin_ctzll(todo) : -1;
+      if (nxt_idx >= 0 && ((pres >> nxt_idx) & 1))
+        read_entry(m, nxt_idx, nxt_idx - i0, buf, va);
+      use_entry(m, idx, (pres >> idx) & 1, vb);
+      if (nxt_idx < 0) break;
+      idx = nxt_idx;
+    }
+  };
+
+  // Metadata blocks of 64 merged entries: `cur` holds the block of the
+  // batch being summed (first entry blk0), `nxt` the following one; a
+  // batch never straddles two blocks (NB divides 64), and the batches being
+  // staged are at most one block ahead.
   SlotMeta<JT> cur =
       load_slot_meta<JT>(m_begin, m_end, k, mrow, mlag, w, dw);
   SlotMeta<JT> nxt =
       load_slot_meta<JT>(m_begin + 64, m_end, k, mrow, mlag, w, dw);
-  int64_t blk0 = m_begin;  // first entry of the metadata block in `cur`
-  stage_load(cur, m_begin, blk0);
-  stage_write(0);
+  int64_t blk0 = m_begin;
+  auto stage = [&](int slot, int64_t mb) {
+    if (mb - blk0 >= 64)
+      stage_load(slot, nxt, mb, blk0 + 64);
+    else
+      stage_load(slot, cur, mb, blk0);
+  };
+  // batch 0 into LDS buffer 0; with DEPTH 2, batch 1's loads in flight
+  stage(0, m_begin);
+  stage_write(0, 0);
+  if (DEPTH == 2 && m_begin + NB < m_end) stage(1 % DEPTH, m_begin + NB);
   __syncthreads();
-  int buf = 0;
-  for (int64_t mb = m_begin; mb < m_end; mb += NB) {
-    const int64_t mb1 = mb + NB;
-    const bool more = mb1 < m_end;
-    const bool new_blk = mb1 - blk0 >= 64;
-    if (more) {
-      if (new_blk)
-        stage_load(nxt, mb1, blk0 + 64);
-      else
-        stage_load(cur, mb1, blk0);
+  // one step: sum batch mb from LDS buffer `buf`; register set `rs_next`
+  // holds the loads of batch mb + NB (issued a step earlier with DEPTH 2,
+  // or now with DEPTH 1), which go to the other LDS buffer afterwards
+  auto step = [&](int64_t mb, int buf, int rs_next) {
+    const int64_t mb1 = mb + NB, mb2 = mb + 2 * NB;
+    if (DEPTH == 1) {
+      if (mb1 < m_end) stage(0, mb1);
+    } else {
+      if (mb2 < m_end) stage(rs_next ^ 1, mb2);
     }
-    compute(cur, mb, blk0, buf);
-    if (more) stage_write(buf ^ 1);
+    if (PIPE)
+      compute_pipe(cur, mb, blk0, buf);
+    else
+      compute(cur, mb, blk0, buf);
+    if (mb1 < m_end) stage_write(DEPTH == 1 ? 0 : rs_next, buf ^ 1);
     __syncthreads();
-    buf ^= 1;
-    if (new_blk) {
+    if (mb1 - blk0 >= 64) {
       cur = nxt;
       blk0 += 64;
       nxt = load_slot_meta<JT>(blk0 + 64, m_end, k, mrow, mlag, w, dw);
     }
+  };
+  // two steps per iteration, so the buffer and register-set indices are
+  // compile-time constants
+  for (int64_t mb = m_begin; mb < m_end; mb += 2 * NB) {
+    step(mb, 0, 1 % DEPTH);
+    if (mb + NB < m_end) step(mb + NB, 1, 0);
   }
   // diffuse + specular plan: the row's diffuse sum over all its entries
   // goes to its diffuse group (the row's last), as ds_gather_kernel stores
@@ -594,7 +704,7 @@ __global__ __launch_bounds__(64 * JT) void tile_lds_gather_kernel(
 // + specular plan, whose second sums take the registers), batches of 16
 // merged entries, windows of TB + 64 bins (2 x 16 x 1,088 floats of LDS
 // at R = 16, one workgroup of 16 waves per CU).
-template <int R, int JT, int NB, int SMAX, bool DS>
+template <int R, int JT, int NB, int SMAX, bool DS, int DEPTH, bool PIPE>
 int launch_tile_lds(int64_t P, int64_t B, int64_t T, int64_t M,
                     const int64_t *tptr, const int64_t *grp_ptr,
                     const int64_t *gdiff, const int32_t *mrow,
@@ -605,7 +715,8 @@ int launch_tile_lds(int64_t P, int64_t B, int64_t T, int64_t M,
   const int64_t nb = n_tiles * B * nt;
   SPR_REQUIRE(nb < (int64_t)0x7fffffff, "exchange grid too large");
   SPR_REQUIRE(T < ((int64_t)1 << 29), "histogram too long (T >= 2^29)");
-  hipLaunchKernelGGL((tile_lds_gather_kernel<R, JT, NB, SMAX, DS>),
+  hipLaunchKernelGGL(
+      (tile_lds_gather_kernel<R, JT, NB, SMAX, DS, DEPTH, PIPE>),
                      dim3((unsigned)nb), dim3(64 * JT), 0, s, P, (int)B,
                      (int)T, (int)nt, nb, M, tptr, grp_ptr, gdiff, mrow,
                      mlag, mw, mdw, prev, gsum);
@@ -718,7 +829,37 @@ extern "C" int spr_exchange_gather_tiled_ds(
   SPR_REQUIRE(tptr && gstart && gdiff && mrow && mlag && mw && mdw && prev &&
                   gsum,
               "spr_exchange_gather_tiled_ds: null pointer");
-  return launch_tile_lds<8, 16, 16, 64, true>(P, B, T, M, tptr, gstart,
-                                               gdiff, mrow, mlag, mw, mdw,
-                                               prev, gsum, as_stream(stream));
+  return launch_tile_lds<8, 16, 16, 64, true, 1, false>(
+      P, B, T, M, tptr, gstart, gdiff, mrow, mlag, mw, mdw, prev, gsum,
+      as_stream(stream));
+}
+
+extern "C" int spr_exchange_gather_tiled_shape(
+    int64_t P, int64_t JT, int64_t R, int64_t B, int64_t T, int64_t M,
+    const int64_t *tptr, const int64_t *grp_ptr, const int32_t *mrow,
+    const int32_t *mlag, const double *mw, const float *prev, float *gsum,
+    void *stream) {
+  SPR_REQUIRE(P > 0 && B > 0 && T > 0 && M >= 0 && JT == 16 &&
+                  (R == 8 || R == 16 || R == -8 || R == -16),
+              "spr_exchange_gather_tiled_shape: bad sizes (JT 16; R 16, 8, "
+              "-8: 8 with two batches in flight, -16: 16 pipelined over "
+              "the present entries)");
+  SPR_REQUIRE(tptr && grp_ptr && mrow && mlag && mw && prev && gsum,
+              "spr_exchange_gather_tiled_shape: null pointer");
+  hipStream_t s = as_stream(stream);
+  if (R == 8)
+    return launch_tile_lds<8, 16, 16, 64, false, 1, false>(
+        P, B, T, M, tptr, grp_ptr, nullptr, mrow, mlag, mw, nullptr, prev,
+        gsum, s);
+  if (R == -16)
+    return launch_tile_lds<16, 16, 16, 64, false, 1, true>(
+        P, B, T, M, tptr, grp_ptr, nullptr, mrow, mlag, mw, nullptr, prev,
+        gsum, s);
+  if (R == -8)
+    return launch_tile_lds<8, 16, 16, 64, false, 2, false>(
+        P, B, T, M, tptr, grp_ptr, nullptr, mrow, mlag, mw, nullptr, prev,
+        gsum, s);
+  return launch_tile_lds<16, 16, 16, 64, false, 1, false>(
+      P, B, T, M, tptr, grp_ptr, nullptr, mrow, mlag, mw, nullptr, prev,
+      gsum, s);
 }

@@ -464,6 +464,8 @@ class TilePlan:
     # diffuse group (-1: none)
     mdw: torch.Tensor = None
     gdiff: torch.Tensor = None
+    # 16-row tiles: bins per lane of the gather's workgroups (16 or 8)
+    R: int = 16
 
 
 def _rows_in_order(plan, order):
@@ -602,11 +604,13 @@ def tile_plan(plan, jt=4, min_merge=1.5, order=None):
     return tp
 
 
-def block_order(patch_points, patch_to_wall_ids, block=(4, 4)):
+def block_order(patch_points, patch_to_wall_ids, block=(4, 4), pad=True):
     """Destination order in which every run of block[0]*block[1] slots is
     one block of neighbouring patches of one wall (the patches of a wall
     form a regular grid; blocks at a wall's edge are padded with empty
-    slots, -1).  For tile_plan(..., order=...)."""
+    slots, -1).  With ``pad=False`` the blocks follow each other without
+    padding (a tile may then hold the end of one block and the start of
+    the next, but no empty slot).  For tile_plan(..., order=...)."""
     pts = np.asarray(patch_points, float)
     wall = np.asarray(patch_to_wall_ids)
     ctr = pts.mean(axis=1)
@@ -628,7 +632,8 @@ def block_order(patch_points, patch_to_wall_ids, block=(4, 4)):
             mask = (bu == key[0]) & (bv == key[1])
             sel = idx[mask][np.lexsort((v[mask], u[mask]))]
             out.extend(sel.tolist())
-            out.extend([-1] * (a * b - len(sel)))
+            if pad:
+                out.extend([-1] * (a * b - len(sel)))
     return np.array(out, np.int64)
 
 
@@ -846,6 +851,12 @@ class ExchangeState:
             return
         if plan.tiles is not None:
             tp = plan.tiles
+            if tp.JT == 16:
+                call("spr_exchange_gather_tiled_shape", tp.P, tp.JT, tp.R,
+                     B, T, tp.M, ptr(tp.tptr), ptr(tp.gstart),
+                     ptr(tp.mrow), ptr(tp.mlag), ptr(tp.mw), ptr(self.prev),
+                     ptr(self._gsum(plan)), stream_ptr())
+                return
             call("spr_exchange_gather_tiled", tp.P, tp.JT, B, T, tp.M,
                  ptr(tp.tptr), ptr(tp.gstart), ptr(tp.mrow),
                  ptr(tp.mlag), ptr(tp.mw), ptr(self.prev),

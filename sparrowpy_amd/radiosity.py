@@ -654,6 +654,12 @@ class DirectionalRadiosityFast:
     # TILE_ROWS consecutive patch ids (a * b must equal TILE_ROWS); None:
     # consecutive ids.
     TILE_BLOCK = None
+    # 16-row tiles: bins per lane of the gather (16; 8; -8: 8 with two
+    # batches of loads in flight)
+    TILE_R = 16
+    # tile blocks padded to whole blocks at wall edges (empty slots), or
+    # run together
+    TILE_PAD = True
 
     def _tile_order(self):
         """Destination order of the tiles (None: consecutive patch ids)."""
@@ -663,7 +669,8 @@ class DirectionalRadiosityFast:
         if a * bb != self.TILE_ROWS:
             raise ValueError("TILE_BLOCK must hold TILE_ROWS patches")
         return ops.block_order(self._patches_points,
-                               self._patch_to_wall_ids, self.TILE_BLOCK)
+                               self._patch_to_wall_ids, self.TILE_BLOCK,
+                               pad=self.TILE_PAD)
 
     def _exchange_plan(self, speed_of_sound, time_resolution, kind=None):
         """Plan of the exchange (cached per speed of sound and bin width).
@@ -674,7 +681,8 @@ class DirectionalRadiosityFast:
         plus one specular entry, else "generic").
         """
         key = (float(speed_of_sound), float(time_resolution), kind,
-               self.TILE_ROWS, self.TILE_MIN_MERGE, self.TILE_BLOCK)
+               self.TILE_ROWS, self.TILE_MIN_MERGE, self.TILE_BLOCK,
+               self.TILE_R, self.TILE_PAD)
         if getattr(self, "_plan", None) is not None and self._plan[0] == key:
             return self._plan[1]
         b = self._bake
@@ -709,6 +717,8 @@ class DirectionalRadiosityFast:
                 plan.tiles = ops.tile_plan(plan, self.TILE_ROWS,
                                            self.TILE_MIN_MERGE,
                                            self._tile_order())
+                if plan.tiles is not None:
+                    plan.tiles.R = self.TILE_R
         if plan is None and kind in (None, "ds"):
             try:
                 plan = ops.diffuse_specular_plan(*args)

@@ -188,38 +188,43 @@ def test_gradient_matches_finite_differences(torch):
 
 
 def test_adam_recovers_materials(torch):
-    """Inverse materials: 200 Adam steps through the adjoint from perturbed
-    absorption / scattering towards the tables that generated the target
-    histograms (4 receivers, 400 bins, 10 orders) cut the log-energy loss
-    by two orders of magnitude and bring the parameters closer to the
-    truth (not all of the 208 parameters of 52 one-patch walls are
-    identifiable from 4 receivers, so the loss falls much faster than the
-    parameter error)."""
+    """Inverse materials: 200 Adam steps through the adjoint towards the
+    histograms of known per-wall absorption and scattering (52 one-patch
+    walls, 2 bands, 4 receivers, 400 bins, 10 orders), starting from
+    every wall's absorption 0.05 too high (plus noise).  Four receivers
+    determine the room's decay, i.e. the per-band mean absorption, not
+    each of the 208 parameters: the loss must fall by two orders of
+    magnitude and the per-band mean absorption error to at most 0.3 of
+    its start (0.045 / 0.041 -> 0.008 / 0.007 when measured); the
+    per-wall absorption error only has to shrink (0.044 -> 0.024)."""
     from sparrowpy_amd import adjoint
     walls, dirs, ta, ts, alpha, scat, src, _, (T, c, dt, K) = _setup(seed=3)
     rcv = np.array([[1.4, 1.0, .6], [.9, .4, .7], [.4, 1.1, .3],
                     [1.6, .3, .8]])
     T, K = 400, 10
     sv, _ = spec_info(dirs)
-    _, adj = _adjoint(torch, walls, dirs, alpha, scat, src, rcv, T, c, dt, K,
-                      5)
+    rng = np.random.default_rng(5)
+    a0 = np.clip(ta + .05 + rng.normal(0, .03, ta.shape), .01, .99)
+    s0 = np.clip(ts + rng.normal(0, .1, ts.shape), .01, .99)
+    _, adj = _adjoint(torch, walls, dirs, a0, s0, src, rcv, T, c, dt, K, 5)
     kw = dict(dtype=torch.float64, device="cuda")
     A, C = adjoint.scattering_tables(torch.tensor(ta, **kw),
                                      torch.tensor(ts, **kw), sv)
     target = adj.forward(A, C).to(torch.float64)
     adj.release()
-    a, s, losses = adjoint.recover_materials(adj, target, alpha, scat, sv,
-                                             steps=200, lr=0.01)
+    a, s, losses = adjoint.recover_materials(adj, target, a0, s0, sv,
+                                             steps=200, lr=0.02)
     a, s = a.cpu().numpy(), s.cpu().numpy()
-    err0 = np.abs(alpha - ta).mean() + np.abs(scat - ts).mean()
-    err1 = np.abs(a - ta).mean() + np.abs(s - ts).mean()
-    print(f"[adam] loss {losses[0]:.3e} -> {losses[-1]:.3e}, "
-          f"|alpha - true| {np.abs(alpha - ta).mean():.4f} -> "
+    band0 = np.abs(a0.mean(0) - ta.mean(0))
+    band1 = np.abs(a.mean(0) - ta.mean(0))
+    print(f"[adam] loss {losses[0]:.3e} -> {losses[-1]:.3e}, band-mean "
+          f"absorption error {band0.round(4)} -> {band1.round(4)}, "
+          f"per-wall |alpha - true| {np.abs(a0 - ta).mean():.4f} -> "
           f"{np.abs(a - ta).mean():.4f}, |s - true| "
-          f"{np.abs(scat - ts).mean():.4f} -> {np.abs(s - ts).mean():.4f}")
+          f"{np.abs(s0 - ts).mean():.4f} -> {np.abs(s - ts).mean():.4f}")
     assert losses[-1] < 1e-2 * losses[0]
-    assert err1 < 0.9 * err0
-
+    assert (band1 <= 0.3 * band0).all()
+    assert np.abs(a - ta).mean() < np.abs(a0 - ta).mean()
 
 
 def test_gradient_without_visible_pairs(torch):

@@ -146,9 +146,11 @@ def test_tiled_gather_bitexact_wide_delays(ops, jt):
     assert torch.equal(st.gsum[:gp.n_groups], ref)
 
 
-@pytest.mark.parametrize("jt,block", [(16, (4, 4)), (8, (2, 4)),
-                                      (4, (2, 2))])
-def test_block_tiles_bitexact(ops, jt, block):
+@pytest.mark.parametrize("jt,block,R,pad", [
+    (16, (4, 4), 16, True), (16, (4, 4), 8, True), (16, (4, 4), -8, True),
+    (16, None, -8, True), (16, (4, 4), -16, True), (16, None, -16, True),
+    (16, (4, 4), 16, False), (8, (2, 4), 16, True), (4, (2, 2), 16, True)])
+def test_block_tiles_bitexact(ops, jt, block, R, pad):
     """Tiles as blocks of neighbouring patches (rows permuted, walls'
     edges padded with empty slots): the same group sums, bit for bit, in
     the plan's own group order (each slot stores at its destination's
@@ -156,10 +158,14 @@ def test_block_tiles_bitexact(ops, jt, block):
     import torch
     r = _room("3x2.5x2")
     gp, _ = _grouped_and_generic(ops, r, 343., 1 / 48000)
-    order = ops.block_order(r.patches_points, r.patch_to_wall_ids, block)
-    assert (order < 0).any()          # some blocks are padded
+    order = None if block is None else ops.block_order(
+        r.patches_points, r.patch_to_wall_ids, block, pad=pad)
+    if block is not None:
+        assert (order < 0).any() == pad   # padded blocks have empty slots
     tp = ops.tile_plan(gp, jt, min_merge=0, order=order)
-    assert tp.P == len(order) + (-len(order)) % jt
+    tp.R = R
+    n = gp.P if order is None else len(order)
+    assert tp.P == n + (-n) % jt
     P, D, B, T = gp.P, gp.D, gp.B, 1500
     st = _random_state(ops, P, D, B, T, seed=5)
     st.gather(gp)
