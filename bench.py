@@ -556,9 +556,8 @@ def main(argv=None):
                          "(e.g. 4x4 with --tile-rows 16); default: the "
                          "class default")
     ap.add_argument("--tile-r", type=int, default=None,
-                    choices=[16, 8, -8, -16],
-                    help="16-row tiles: bins per lane of the gather (-8: 8 "
-                         "with two batches of loads in flight)")
+                    choices=[16, 8],
+                    help="16-row tiles: bins per lane of the gather")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a device copy of known size (to scale "
                          "the FETCH_SIZE/WRITE_SIZE counters)")

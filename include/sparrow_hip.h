@@ -412,14 +412,22 @@ int spr_plan_tile_fill(int64_t P, int64_t JT, int64_t B, int64_t Vd,
                        const int64_t *payload, const int64_t *first,
                        int32_t *mrow, int32_t *mlag, double *mw, void *stream);
 
+/* Pass 3 (16-row tiles): mspan[M,4] (int32) = {source row, largest and
+ * smallest delay of the merged entry's slots, 0}, the window each merged
+ * entry's source row is staged over by the 16-row gathers.
+ * Replaces, with spr_plan_tile_order, the per-pair terms of one order of
+ * _energy_exchange, RadiosityFast.py:1121-1144. */
+int spr_plan_tile_span(int64_t M, int64_t JT, const int32_t *mrow,
+                       const int32_t *mlag, int32_t *mspan, void *stream);
+
 /* Gather pass on the tiles: gsum bit-identical to
  * spr_exchange_gather_grouped (each slot sums its groups in the same
  * order), for use with spr_exchange_spread_grouped.  JT = 4 or 8: the
  * slots' loads of a merged source row are issued together and share cache
- * lines; JT = 16: one wave per slot, each merged row's window staged once
- * in LDS.  grp_ptr[slot] = index of the slot's first group (the plan's
- * grp_ptr for consecutive rows; for permuted rows, that of the slot's
- * destination); P = number of slots.
+ * lines (16-row tiles: spr_exchange_gather_tiled_shape).  grp_ptr[slot] =
+ * index of the slot's first group (the plan's grp_ptr for consecutive
+ * rows; for permuted rows, that of the slot's destination); P = number of
+ * slots.
  * Replaces, with a spread, one order of _energy_exchange,
  * RadiosityFast.py:1121-1144. */
 int spr_exchange_gather_tiled(int64_t P, int64_t JT, int64_t B, int64_t T,
@@ -428,17 +436,18 @@ int spr_exchange_gather_tiled(int64_t P, int64_t JT, int64_t B, int64_t T,
                               const int32_t *mlag, const double *mw,
                               const float *prev, float *gsum, void *stream);
 
-/* spr_exchange_gather_tiled on 16-row tiles with a chosen workgroup shape:
- * R = 16 bins per lane (1,024-bin blocks, one workgroup per CU) or 8
- * (512-bin blocks, two workgroups per CU); the same group sums, bit for
- * bit.  For autotuning.
+/* spr_exchange_gather_tiled on 16-row tiles: one wave per slot, each
+ * merged entry's window of its source row (mspan, spr_plan_tile_span)
+ * staged once in LDS with the batch's delay words and weights.  R = 16
+ * bins per lane (1,024-bin blocks) or 8 (512-bin blocks); the same group
+ * sums, bit for bit.
  * Replaces, with a spread, one order of _energy_exchange,
  * RadiosityFast.py:1121-1144. */
 int spr_exchange_gather_tiled_shape(int64_t P, int64_t JT, int64_t R,
                                     int64_t B, int64_t T, int64_t M,
                                     const int64_t *tptr,
                                     const int64_t *grp_ptr,
-                                    const int32_t *mrow, const int32_t *mlag,
+                                    const int32_t *mspan, const int32_t *mlag,
                                     const double *mw, const float *prev,
                                     float *gsum, void *stream);
 
@@ -448,13 +457,13 @@ int spr_exchange_gather_tiled_shape(int64_t P, int64_t JT, int64_t R,
  * mdw[B,M,16]: specular and diffuse weights per slot (two
  * spr_plan_tile_fill passes, with scal and with dscal); gstart[slot],
  * gdiff[slot] (int64): the slot's first group and its diffuse group (-1:
- * the slot's row has no entries).
+ * the slot's row has no entries); mspan from spr_plan_tile_span.
  * Replaces, with spr_exchange_spread_ds, one order of _energy_exchange,
  * RadiosityFast.py:1121-1144. */
 int spr_exchange_gather_tiled_ds(int64_t P, int64_t JT, int64_t B, int64_t T,
                                  int64_t M, const int64_t *tptr,
                                  const int64_t *gstart, const int64_t *gdiff,
-                                 const int32_t *mrow, const int32_t *mlag,
+                                 const int32_t *mspan, const int32_t *mlag,
                                  const double *mw, const double *mdw,
                                  const float *prev, float *gsum,
                                  void *stream);

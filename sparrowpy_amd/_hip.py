@@ -107,6 +107,7 @@ SYMBOLS = {
     "spr_plan_tile_fill": (_int, [_i64, _i64, _i64, _i64, _i64, _vp, _vp,
                                   _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
                                   _vp]),
+    "spr_plan_tile_span": (_int, [_i64, _i64, _vp, _vp, _vp, _vp]),
     "spr_exchange_gather_tiled": (_int, [_i64, _i64, _i64, _i64, _i64, _vp,
                                          _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "spr_exchange_gather_tiled_shape": (_int, [_i64, _i64, _i64, _i64,

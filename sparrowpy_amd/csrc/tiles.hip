@@ -343,13 +343,6 @@ int launch_tile_shape(int64_t P, int64_t B, int64_t T, int64_t M,
   return SPR_OK;
 }
 
-template <int R, int JT, int NB, int SMAX, bool DS, int DEPTH, bool PIPE>
-int launch_tile_lds(int64_t P, int64_t B, int64_t T, int64_t M,
-                    const int64_t *tptr, const int64_t *grp_ptr,
-                    const int64_t *gdiff, const int32_t *mrow,
-                    const int32_t *mlag, const double *mw, const double *mdw,
-                    const float *prev, float *gsum, hipStream_t s);
-
 int launch_tile_gather(int64_t P, int JT, int64_t B, int64_t T, int64_t M,
                        const int64_t *tptr, const int64_t *grp_ptr,
                        const int32_t *mrow, const int32_t *mlag,
@@ -358,12 +351,8 @@ int launch_tile_gather(int64_t P, int JT, int64_t B, int64_t T, int64_t M,
   if (JT == 4)
     return launch_tile_shape<4, 1, 4, 4, 64>(P, B, T, M, tptr, grp_ptr, mrow,
                                              mlag, mw, prev, gsum, s);
-  if (JT == 8)
-    return launch_tile_shape<4, 1, 8, 4, 64>(P, B, T, M, tptr, grp_ptr,
-                                             mrow, mlag, mw, prev, gsum, s);
-  return launch_tile_lds<16, 16, 16, 64, false, 1, false>(
-      P, B, T, M, tptr, grp_ptr, nullptr, mrow, mlag, mw, nullptr, prev, gsum,
-      s);
+  return launch_tile_shape<4, 1, 8, 4, 64>(P, B, T, M, tptr, grp_ptr, mrow,
+                                           mlag, mw, prev, gsum, s);
 }
 
 // ---------------------------------------------------------------------
@@ -378,81 +367,97 @@ int launch_tile_gather(int64_t P, int JT, int64_t B, int64_t T, int64_t M,
 // view of it from LDS.
 //
 // A workgroup owns (tile, band, block of TB = 64*R bins).  Merged entries
-// are taken in batches of NB; for each, the window of source bins
+// are taken in batches of NB = JT; for each, the window of source bins
 // [t0 - maxlag, t0 - maxlag + W) with W = TB + SMAX covers every present
 // slot whose delay is within SMAX of the entry's largest delay (maxlag); a
 // slot with a larger spread reads its bins from global memory instead
-// (uniform branch, rare).  Two LDS buffers: the global loads of batch q+1
-// are in flight while batch q is summed from LDS; one barrier per batch.
-// Each wave sums its slot's entries in merged order, which within a group
-// is the grouped plan's entry order: the per-group sums are bit-identical
-// to spr_exchange_gather_grouped's.  Requires T < 2^29.
-template <int JT>
-struct SlotMeta {
-  int32_t row;     // source row of the merged entry
-  int32_t lag;     // this wave's slot delay word (kLagMask: absent)
-  int32_t maxlag;  // largest delay of the entry's present slots
-  int32_t minlag;  // smallest
-  double w;        // this wave's slot weight in the workgroup's band
-  double dw;       // its diffuse weight (diffuse + specular plan)
-};
+// (uniform branch, rare).  The batch's metadata goes to LDS beside the
+// windows: per (entry, slot) the LDS offset of the slot's view (maxlag -
+// delay, with the delay word's flags) and the weights, per entry its
+// source row and maxlag.  A wave reads its slot's 16 offset words with one
+// LDS load (one per lane) and takes each with v_readlane, and reads each
+// weight with a broadcast LDS load straight into the FMA's operand, so a
+// present entry costs the wave one v_readlane, one address add, R LDS
+// reads, R conversions and R FP64 FMAs.  Two LDS buffers: the global loads
+// of batch q+1 are in flight while batch q is summed from LDS; one barrier
+// per batch.  Each wave sums its slot's entries in merged order, which
+// within a group is the grouped plan's entry order: the per-group sums are
+// bit-identical to spr_exchange_gather_grouped's.  Requires T < 2^29.
+typedef __attribute__((address_space(3))) void lds_void_t;
 
-// Merged entries m0 .. m0+63, one per lane, as seen by slot k.
-template <int JT>
-__device__ __forceinline__ SlotMeta<JT> load_slot_meta(
-    int64_t m0, int64_t m_end, int k, const int32_t *__restrict__ mrow,
-    const int32_t *__restrict__ mlag, const double *__restrict__ w,
-    const double *__restrict__ dw) {
-  const int64_t m = m0 + (int64_t)(threadIdx.x & 63);
-  const bool in = m < m_end;
-  const int64_t mc = in ? m : m_end - 1;
-  SlotMeta<JT> t;
-  t.row = mrow[mc];
-  int32_t mx = 0, mn = kLagMask;
-  const int4 *lw = reinterpret_cast<const int4 *>(mlag + mc * JT);
-#pragma unroll
-  for (int q = 0; q < JT / 4; ++q) {
-    const int4 v = lw[q];
-    const int32_t a[4] = {v.x, v.y, v.z, v.w};
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      const int32_t l = a[u] & kLagMask;
-      if (l != kLagMask) {
-        mx = max(mx, l);
-        mn = min(mn, l);
-      }
-    }
-  }
-  t.maxlag = mx;
-  t.minlag = mn == kLagMask ? mx : mn;
-  t.lag = in ? mlag[mc * JT + k] : kLagMask;
-  t.w = in ? w[mc * JT + k] : 0.0;
-  t.dw = in && dw ? dw[mc * JT + k] : 0.0;
-  return t;
+// One buffer load of a dword per lane into LDS: lane l's dword goes to
+// dst + 4 l (+ the immediate offset, which also moves the memory address).
+__device__ __forceinline__ void dma_dword(__amdgpu_buffer_rsrc_t rs,
+                                          float *dst, uint32_t off) {
+  __builtin_amdgcn_raw_ptr_buffer_load_lds(rs, (lds_void_t *)dst, 4, off, 0, 0,
+                                           0);
 }
 
-template <int R, int JT, int NB, int SMAX, bool DS, int DEPTH, bool PIPE>
-__global__ __launch_bounds__(64 * JT) void tile_lds_gather_kernel(
+// Chunks X .. N-1 of 64 dwords, chunk x at immediate offset 256 x (a
+// constant expression, as the instruction field must be).
+template <int X, int N>
+struct WindowDma {
+  __device__ __forceinline__ static void run(__amdgpu_buffer_rsrc_t rs,
+                                             float *dst, uint32_t vo) {
+    __builtin_amdgcn_raw_ptr_buffer_load_lds(rs, (lds_void_t *)dst, 4, vo, 0,
+                                             256 * X, 0);
+    WindowDma<X + 1, N>::run(rs, dst, vo);
+  }
+};
+template <int N>
+struct WindowDma<N, N> {
+  __device__ __forceinline__ static void run(__amdgpu_buffer_rsrc_t,
+                                             float *, uint32_t) {}
+};
+
+template <int NB, int JT, bool DS>
+struct BatchMeta {
+  int32_t word[NB][JT];       // (maxlag - delay) | flags; kLagMask | flags
+  double w[NB][JT];           // weights in the workgroup's band
+  double dw[DS ? NB : 1][JT]; // diffuse weights (diffuse + specular plan)
+  int32_t row[NB];            // source row
+  int32_t maxlag[NB];         // largest delay of the entry's present slots
+};
+
+// R = 8 without the diffuse sums: 80 KiB of LDS and at most 64 VGPRs, so
+// two workgroups share a CU (8 waves per SIMD): the sums are latency-bound
+// (each entry's reads, conversions and FP64 FMAs form one chain per wave),
+// and twice the waves hide twice the latency.
+template <int R, bool DS>
+constexpr int lds_gather_waves_per_eu() {
+  return R == 8 && !DS ? 8 : 1;
+}
+
+template <int R, int JT, int SMAX, bool DS>
+__global__ __launch_bounds__(64 * JT) __attribute__((
+    amdgpu_waves_per_eu(lds_gather_waves_per_eu<R, DS>()))) void
+tile_lds_gather_kernel(
     int64_t P, int B, int T, int n_tblk, int64_t nblocks, int64_t M,
     const int64_t *__restrict__ tptr, const int64_t *__restrict__ grp_ptr,
-    const int64_t *__restrict__ gdiff, const int32_t *__restrict__ mrow,
+    const int64_t *__restrict__ gdiff, const int4 *__restrict__ mspan,
     const int32_t *__restrict__ mlag, const double *__restrict__ mw,
     const double *__restrict__ mdw, const float *__restrict__ prev,
     float *__restrict__ gsum) {
+  constexpr int NB = JT;           // entries per batch: one window per wave
   constexpr int TB = 64 * R;       // bins of the workgroup
   constexpr int W = TB + SMAX;     // LDS window of one merged entry
   constexpr int NX = W / 64;       // window floats per lane
-  constexpr int NE = NB / JT;      // entries each wave stages per batch
-  static_assert(SMAX % 64 == 0 && NB % JT == 0 && 64 % NB == 0, "shape");
-  static_assert(DEPTH == 1 || DEPTH == 2, "one or two batches in flight");
-  static_assert(64 / NB >= DEPTH, "staging stays within the next block");
+  constexpr int NM = NB * JT;      // (entry, slot) pairs of a batch
+  static_assert(SMAX % 64 == 0 && NM % 64 == 0 &&
+                    (DS ? 3 : 2) * NM <= 64 * JT && NB <= 32 &&
+                    (NB & (NB - 1)) == 0,
+                "shape: whole waves stage the metadata");
   __shared__ float lds[2][NB][W];
+  __shared__ BatchMeta<NB, JT, DS> meta[2];
   int64_t tile;
   int b, tb;
   dest_major_tile(xcd_tile(blockIdx.x, nblocks), nblocks / (B * n_tblk),
                   n_tblk, tile, b, tb);
-  const int64_t m_begin = tptr[tile], m_end = tptr[tile + 1];
-  if (m_end <= m_begin) return;  // uniform: before any barrier
+  const int64_t m_begin = tptr[tile];
+  // entries of the tile (< 2^31): batch positions are 32-bit offsets from
+  // m_begin, so every loop bound is a scalar 32-bit compare
+  const int n_tile = (int)(tptr[tile + 1] - m_begin);
+  if (n_tile <= 0) return;  // uniform: before any barrier
   const int t0 = tb * TB;
   // slot = wave; readfirstlane makes it (and everything derived from it:
   // the slot's row, its group counter, its branches) a scalar value, so
@@ -462,43 +467,85 @@ __global__ __launch_bounds__(64 * JT) void tile_lds_gather_kernel(
   const int64_t j = tile * JT + k;
   const bool active = j < P;
   int64_t g = active ? grp_ptr[j] : 0;
-  const double *w = mw + (int64_t)b * M * JT;
-  const double *dw = DS ? mdw + (int64_t)b * M * JT : nullptr;
+  const double *w = mw + ((int64_t)b * M + m_begin) * JT;
+  const double *dw = DS ? mdw + ((int64_t)b * M + m_begin) * JT : nullptr;
+  const int32_t *lagw = mlag + m_begin * JT;
+  const int4 *span = mspan + m_begin;
   const uint32_t row_bytes = (uint32_t)T * 4u;
   const int64_t bstride = (int64_t)B * T;
+  auto n_at = [&](int mb) { return min(NB, n_tile - mb); };
 
-  // batch staging: wave k loads entries k, k + JT, ... of a batch into
-  // registers st[slot]; DEPTH register sets, so with DEPTH = 2 the loads of
-  // batch q+2 are issued while batch q+1's are still in flight
-  float st[DEPTH][NE][NX];
-  auto stage_load = [&](int slot, const SlotMeta<JT> &m, int64_t mb,
-                        int64_t blk0) {
+  // window staging: wave k copies entry mb + k's window of its source row
+  // straight from memory into LDS (buffer loads with the LDS flag: lane l
+  // of a load writes dword l at the load's LDS address), from the entry's
+  // span (source row, maxlag, minlag: a scalar load, one batch ahead).  A
+  // window starting at or after the row's first bin (w0 >= 0) takes a
+  // descriptor based at bin w0, so the 64-bin steps are immediate offsets
+  // (added to both the memory and the LDS address) that cannot wrap; one
+  // starting before it keeps each lane's offset whole (bin_offset),
+  // negative bins reading 0.  Nothing passes through registers: the loads
+  // are waited for (vmcnt) before the barrier that publishes the buffer.
+  auto span_of = [&](int mm) {
+    return mm < n_tile ? span[mm] : make_int4(0, 0, 0, 0);
+  };
+  auto stage_load = [&](int buf, int n_in, const int4 &sp) {
+    if (k >= n_in) return;  // uniform
+    const int need = min(W, TB + (sp.y - sp.z));
+    const float *row = prev + (int64_t)sp.x * bstride + (int64_t)b * T;
+    const int w0 = t0 - sp.y;
+    float *dst = &lds[buf][k][0];
+    if (w0 >= 0) {
+      const __amdgpu_buffer_rsrc_t rs = make_rsrc(
+          row + w0, w0 < T ? (uint32_t)(T - w0) * 4u : 0u);
+      const uint32_t vo = (uint32_t)lane * 4u;
+      WindowDma<0, (NX < 16 ? NX : 16)>::run(rs, dst, vo);
 #pragma unroll
-    for (int e = 0; e < NE; ++e) {
-      const int64_t mm = mb + k + e * JT;
-      const int idx = (int)(mm - blk0);
-      const bool have = mm < m_end;
-      const int32_t row = __builtin_amdgcn_readlane(m.row, have ? idx : 0);
-      const int32_t mx = __builtin_amdgcn_readlane(m.maxlag, have ? idx : 0);
-      const int32_t mn = __builtin_amdgcn_readlane(m.minlag, have ? idx : 0);
-      const int need = have ? min(W, TB + (mx - mn)) : 0;
-      const __amdgpu_buffer_rsrc_t rs =
-          make_rsrc(prev + (int64_t)row * bstride + (int64_t)b * T,
-                    row_bytes);
-      const int s0 = t0 - mx + lane;
+      for (int x = 16; x < NX; ++x)
+        if (x < TB / 64 || 64 * x < need)
+          dma_dword(rs, dst + 64 * x, vo + 256u * (uint32_t)x);
+    } else {
+      const __amdgpu_buffer_rsrc_t rs = make_rsrc(row, row_bytes);
 #pragma unroll
       for (int x = 0; x < NX; ++x)
-        st[slot][e][x] = 64 * x < need
-                             ? buffer_load(rs, bin_offset(s0 + 64 * x))
-                             : 0.0f;
+        if (x < TB / 64 || 64 * x < need)
+          dma_dword(rs, dst + 64 * x, bin_offset(w0 + lane + 64 * x));
+    }
+    if (lane == 0) {
+      meta[buf].row[k] = sp.x;
+      meta[buf].maxlag[k] = sp.y;
     }
   };
-  auto stage_write = [&](int slot, int buf) {
-#pragma unroll
-    for (int e = 0; e < NE; ++e)
-#pragma unroll
-      for (int x = 0; x < NX; ++x)
-        lds[buf][k + e * JT][lane + 64 * x] = st[slot][e][x];
+  // metadata staging: waves 0 .. NM/64-1 take the delay words (pair q =
+  // (e, kk) = (q / JT, q % JT)), the next NM/64 waves the weights, and for
+  // the diffuse + specular plan the next NM/64 the diffuse weights.  Every
+  // thread issues the same loads (entry index clamped into the batch,
+  // weight array chosen per wave), without branches: the values are only
+  // used after the sums, so nothing waits for them (or, vmcnt being in
+  // order, for the window copies issued before them) in between.  Pairs
+  // past the batch's end are staged with clamped values the sums never
+  // read.
+  const int q_part = k / (NM / 64);
+  const int q = (int)threadIdx.x - q_part * NM;
+  const int q_e = q / JT, q_k = q % JT;
+  const double *wsrc = DS && q_part == 2 ? dw : w;
+  int32_t m_word = 0, m_max = 0;
+  double m_w = 0.0;
+  auto meta_load = [&](int mb, int n_in) {
+    const int64_t mm = (int64_t)(mb + min(q_e, n_in - 1));
+    m_word = lagw[mm * JT + q_k];
+    m_max = reinterpret_cast<const int32_t *>(span)[mm * 4 + 1];
+    m_w = wsrc[mm * JT + q_k];
+  };
+  auto meta_write = [&](int buf) {
+    if (q_part == 0) {
+      const int32_t l = m_word & kLagMask;
+      meta[buf].word[q_e][q_k] =
+          (l == kLagMask ? kLagMask : m_max - l) | (m_word & kLastInGroup);
+    } else if (q_part == 1) {
+      meta[buf].w[q_e][q_k] = m_w;
+    } else if (DS && q_part == 2) {
+      meta[buf].dw[DS ? q_e : 0][q_k] = m_w;
+    }
   };
 
   double S[R], Sd[DS ? R : 1];
@@ -506,98 +553,65 @@ __global__ __launch_bounds__(64 * JT) void tile_lds_gather_kernel(
   for (int r = 0; r < R; ++r) S[r] = 0.0;
 #pragma unroll
   for (int r = 0; r < (DS ? R : 1); ++r) Sd[r] = 0.0;
-  auto compute = [&](const SlotMeta<JT> &m, int64_t mb, int64_t blk0,
-                     int buf) {
+
+  // The sums of one batch, software-pipelined: the wave walks the entries
+  // where its slot is present or ends a group (set bits of two ballots
+  // over its 16 offset words, one per lane), in order, and issues the next
+  // one's LDS reads (values and weights) before summing the current one.
+  // The reads are unconditional (an absent entry, or one whose delay is
+  // beyond the window, reads an in-window view it does not use), so the
+  // sums wait only for the reads of their own entry.
+  struct Cur {
+    float v[R];
+    double w, dw;
+    int32_t lw;
+  };
+  auto read_entry = [&](int buf, int idx, int32_t words, Cur &c) {
+    c.lw = __builtin_amdgcn_readlane(words, idx);
+    const int off = min(c.lw & kLagMask, SMAX);
+    const float *src = &lds[buf][idx][off + lane];
 #pragma unroll
-    for (int e = 0; e < NB; ++e) {
-      const int64_t mm = mb + e;
-      if (mm >= m_end) break;  // uniform
-      const int idx = (int)(mm - blk0);
-      const int32_t lw = __builtin_amdgcn_readlane(m.lag, idx);
-      const int32_t l = lw & kLagMask;
-      if (active && l != kLagMask) {
-        const double wk = readlane_f64(m.w, idx);
-        const int32_t mx = __builtin_amdgcn_readlane(m.maxlag, idx);
-        const int off = mx - l;
-        float v[R];
-        if (off <= SMAX) {
+    for (int r = 0; r < R; ++r) c.v[r] = src[64 * r];
+    c.w = meta[buf].w[idx][k];
+    if (DS) c.dw = meta[buf].dw[DS ? idx : 0][k];
+  };
+  auto sum_entry = [&](const float (&v)[R], double wk, double dk) {
+    if (DS) {
 #pragma unroll
-          for (int r = 0; r < R; ++r) v[r] = lds[buf][e][off + lane + 64 * r];
-        } else {  // delay far from the entry's others: global memory
-          const int32_t row = __builtin_amdgcn_readlane(m.row, idx);
-          const __amdgpu_buffer_rsrc_t rs =
-              make_rsrc(prev + (int64_t)row * bstride + (int64_t)b * T,
-                        row_bytes);
-#pragma unroll
-          for (int r = 0; r < R; ++r)
-            v[r] = buffer_load(rs, bin_offset(t0 + lane + 64 * r - l));
-        }
-        if (DS) {
-          const double dk = readlane_f64(m.dw, idx);
-#pragma unroll
-          for (int r = 0; r < R; ++r) {
-            const double x = (double)v[r];
-            S[r] = fma(wk, x, S[r]);
-            Sd[DS ? r : 0] = fma(dk, x, Sd[DS ? r : 0]);
-          }
-        } else {
-#pragma unroll
-          for (int r = 0; r < R; ++r) S[r] = fma(wk, (double)v[r], S[r]);
-        }
+      for (int r = 0; r < R; ++r) {
+        const double x = (double)v[r];
+        S[r] = fma(wk, x, S[r]);
+        Sd[DS ? r : 0] = fma(dk, x, Sd[DS ? r : 0]);
       }
-      if (active && (lw & kLastInGroup)) {
-        const __amdgpu_buffer_rsrc_t os =
-            make_rsrc(gsum + (g * B + b) * (int64_t)T, row_bytes);
+    } else {
 #pragma unroll
-        for (int r = 0; r < R; ++r) {
-          buffer_store((float)S[r], os, (uint32_t)(t0 + lane + 64 * r) * 4u);
-          S[r] = 0.0;
-        }
-        g += 1;
-      }
+      for (int r = 0; r < R; ++r) S[r] = fma(wk, (double)v[r], S[r]);
     }
   };
-
-  // The same sums, software-pipelined: the wave walks only the entries of
-  // the batch where its slot is present or ends a group (set bits of a
-  // ballot over the metadata lanes), and issues the next present entry's
-  // LDS reads into the other register set before summing the current one.
-  auto read_entry = [&](const SlotMeta<JT> &m, int idx, int e, int buf,
-                        float (&v)[R]) {
-    const int32_t l = __builtin_amdgcn_readlane(m.lag, idx) & kLagMask;
-    const int32_t mx = __builtin_amdgcn_readlane(m.maxlag, idx);
-    const int off = mx - l;
+  auto use_entry = [&](int buf, int idx, const Cur &c) {
+    const int32_t off = c.lw & kLagMask;
     if (off <= SMAX) {
-#pragma unroll
-      for (int r = 0; r < R; ++r) v[r] = lds[buf][e][off + lane + 64 * r];
-    } else {
-      const int32_t row = __builtin_amdgcn_readlane(m.row, idx);
+      sum_entry(c.v, c.w, c.dw);
+    } else if (off != kLagMask) {
+      // delay far from the entry's others: its bins from global memory,
+      // into registers of their own (the prefetched view stays untouched)
+      const int32_t row = __builtin_amdgcn_readfirstlane(meta[buf].row[idx]);
+      const int32_t l =
+          __builtin_amdgcn_readfirstlane(meta[buf].maxlag[idx]) - off;
       const __amdgpu_buffer_rsrc_t rs = make_rsrc(
           prev + (int64_t)row * bstride + (int64_t)b * T, row_bytes);
+      float v[R];
 #pragma unroll
       for (int r = 0; r < R; ++r)
         v[r] = buffer_load(rs, bin_offset(t0 + lane + 64 * r - l));
+      // drain here (vmcnt(0); expcnt and lgkmcnt left alone): with loads
+      // of this path still pending at the join, the compiler would make
+      // later reads into the same registers wait for vmcnt, and with it
+      // for the next batch's window copies
+      __builtin_amdgcn_s_waitcnt(0x0f70);
+      sum_entry(v, c.w, c.dw);
     }
-  };
-  auto use_entry = [&](const SlotMeta<JT> &m, int idx, bool present,
-                       const float (&v)[R]) {
-    const int32_t lw = __builtin_amdgcn_readlane(m.lag, idx);
-    if (present) {
-      const double wk = readlane_f64(m.w, idx);
-      if (DS) {
-        const double dk = readlane_f64(m.dw, idx);
-#pragma unroll
-        for (int r = 0; r < R; ++r) {
-          const double x = (double)v[r];
-          S[r] = fma(wk, x, S[r]);
-          Sd[DS ? r : 0] = fma(dk, x, Sd[DS ? r : 0]);
-        }
-      } else {
-#pragma unroll
-        for (int r = 0; r < R; ++r) S[r] = fma(wk, (double)v[r], S[r]);
-      }
-    }
-    if (lw & kLastInGroup) {
+    if (c.lw & kLastInGroup) {
       const __amdgpu_buffer_rsrc_t os =
           make_rsrc(gsum + (g * B + b) * (int64_t)T, row_bytes);
 #pragma unroll
@@ -608,85 +622,68 @@ __global__ __launch_bounds__(64 * JT) void tile_lds_gather_kernel(
       g += 1;
     }
   };
-  auto compute_pipe = [&](const SlotMeta<JT> &m, int64_t mb, int64_t blk0,
-                          int buf) {
-    if (!active) return;  // uniform per wave
-    const int i0 = (int)(mb - blk0);
-    const int n = (int)min((int64_t)NB, m_end - mb);
-    const uint64_t window = (n >= 64 ? ~0ull : ((1ull << n) - 1ull)) << i0;
-    const int32_t lw = m.lag;  // this lane's entry of the block
-    const uint64_t pres =
-        __ballot((lw & kLagMask) != kLagMask) & window;
-    uint64_t todo = (pres | __ballot((lw & kLastInGroup) != 0)) & window;
-    if (!todo) return;
-    float va[R], vb[R];
-    int idx = __builtin_ctzll(todo);
-    if ((pres >> idx) & 1) read_entry(m, idx, idx - i0, buf, va);
-    while (true) {
-      todo &= todo - 1;
-      int nxt_idx = todo ? __builtin_ctzll(todo) : -1;
-      if (nxt_idx >= 0 && ((pres >> nxt_idx) & 1))
-        read_entry(m, nxt_idx, nxt_idx - i0, buf, vb);
-      use_entry(m, idx, (pres >> idx) & 1, va);
-      if (nxt_idx < 0) break;
-      idx = nxt_idx;
-      todo &= todo - 1;
-      nxt_idx = todo ? __builtin_ctzll(todo) : -1;
-      if (nxt_idx >= 0 && ((pres >> nxt_idx) & 1))
-        read_entry(m, nxt_idx, nxt_idx - i0, buf, va);
-      use_entry(m, idx, (pres >> idx) & 1, vb);
-      if (nxt_idx < 0) break;
-      idx = nxt_idx;
+  auto compute = [&](int buf, int n_in) {
+    // lane e (mod NB) holds entry e's offset word of this slot
+    const int32_t words = meta[buf].word[lane & (NB - 1)][k];
+    const uint32_t in_mask = (uint32_t)((1ull << n_in) - 1ull);
+    const uint32_t pres =
+        (uint32_t)__ballot((words & kLagMask) != kLagMask) & in_mask;
+    const uint32_t ends =
+        (uint32_t)__ballot((words & kLastInGroup) != 0) & in_mask;
+    uint32_t todo = __builtin_amdgcn_readfirstlane(pres | ends);
+    const int n_todo = __builtin_popcount(todo);
+    if (n_todo == 0) return;
+    // i-th entry of the walk in ca (i even) or cb (i odd); the next one's
+    // reads are issued before the current one is summed (at the last, a
+    // harmless second read of the same entry)
+    Cur ca, cb;
+    int idx = __builtin_ctz(todo);
+    read_entry(buf, idx, words, ca);
+#pragma unroll
+    for (int i = 0; i < NB; i += 2) {
+      if (i >= n_todo) break;  // uniform
+      todo &= todo - 1u;
+      int nx = todo ? __builtin_ctz(todo) : idx;
+      read_entry(buf, nx, words, cb);
+      use_entry(buf, idx, ca);
+      idx = nx;
+      if (i + 1 >= n_todo) break;
+      todo &= todo - 1u;
+      nx = todo ? __builtin_ctz(todo) : idx;
+      read_entry(buf, nx, words, ca);
+      use_entry(buf, idx, cb);
+      idx = nx;
     }
   };
 
-  // Metadata blocks of 64 merged entries: `cur` holds the block of the
-  // batch being summed (first entry blk0), `nxt` the following one; a
-  // batch never straddles two blocks (NB divides 64), and the batches being
-  // staged are at most one block ahead.
-  SlotMeta<JT> cur =
-      load_slot_meta<JT>(m_begin, m_end, k, mrow, mlag, w, dw);
-  SlotMeta<JT> nxt =
-      load_slot_meta<JT>(m_begin + 64, m_end, k, mrow, mlag, w, dw);
-  int64_t blk0 = m_begin;
-  auto stage = [&](int slot, int64_t mb) {
-    if (mb - blk0 >= 64)
-      stage_load(slot, nxt, mb, blk0 + 64);
-    else
-      stage_load(slot, cur, mb, blk0);
-  };
-  // batch 0 into LDS buffer 0; with DEPTH 2, batch 1's loads in flight
-  stage(0, m_begin);
-  stage_write(0, 0);
-  if (DEPTH == 2 && m_begin + NB < m_end) stage(1 % DEPTH, m_begin + NB);
+  // batch 0 into buffer 0; sp holds the span of this wave's entry of the
+  // next batch to stage
+  int4 sp = span_of(k);
+  stage_load(0, n_at(0), sp);
+  meta_load(0, n_at(0));
+  meta_write(0);
+  sp = span_of(NB + k);
+  __builtin_amdgcn_s_waitcnt(0x0f70);  // vmcnt(0): this wave's window copies
   __syncthreads();
-  // one step: sum batch mb from LDS buffer `buf`; register set `rs_next`
-  // holds the loads of batch mb + NB (issued a step earlier with DEPTH 2,
-  // or now with DEPTH 1), which go to the other LDS buffer afterwards
-  auto step = [&](int64_t mb, int buf, int rs_next) {
-    const int64_t mb1 = mb + NB, mb2 = mb + 2 * NB;
-    if (DEPTH == 1) {
-      if (mb1 < m_end) stage(0, mb1);
-    } else {
-      if (mb2 < m_end) stage(rs_next ^ 1, mb2);
+  // one step: sum batch mb from LDS buffer `buf` while batch mb + NB's
+  // loads are in flight; they go to the other buffer afterwards
+  auto step = [&](int mb, int buf) {
+    const int n1 = n_at(mb + NB);
+    if (n1 > 0) {
+      stage_load(buf ^ 1, n1, sp);
+      meta_load(mb + NB, n1);
+      sp = span_of(mb + 2 * NB + k);
     }
-    if (PIPE)
-      compute_pipe(cur, mb, blk0, buf);
-    else
-      compute(cur, mb, blk0, buf);
-    if (mb1 < m_end) stage_write(DEPTH == 1 ? 0 : rs_next, buf ^ 1);
+    if (active) compute(buf, n_at(mb));
+    if (n1 > 0) meta_write(buf ^ 1);
+    __builtin_amdgcn_s_waitcnt(0x0f70);  // vmcnt(0): this wave's window copies
     __syncthreads();
-    if (mb1 - blk0 >= 64) {
-      cur = nxt;
-      blk0 += 64;
-      nxt = load_slot_meta<JT>(blk0 + 64, m_end, k, mrow, mlag, w, dw);
-    }
   };
-  // two steps per iteration, so the buffer and register-set indices are
-  // compile-time constants
-  for (int64_t mb = m_begin; mb < m_end; mb += 2 * NB) {
-    step(mb, 0, 1 % DEPTH);
-    if (mb + NB < m_end) step(mb + NB, 1, 0);
+  // two steps per iteration, so the buffer indices are compile-time
+  // constants
+  for (int mb = 0; mb < n_tile; mb += 2 * NB) {
+    step(mb, 0);
+    if (mb + NB < n_tile) step(mb + NB, 1);
   }
   // diffuse + specular plan: the row's diffuse sum over all its entries
   // goes to its diffuse group (the row's last), as ds_gather_kernel stores
@@ -700,14 +697,19 @@ __global__ __launch_bounds__(64 * JT) void tile_lds_gather_kernel(
   }
 }
 
-// 16-row tiles: R = 16 bins per lane (1,024-bin blocks; 8 for the diffuse
-// + specular plan, whose second sums take the registers), batches of 16
-// merged entries, windows of TB + 64 bins (2 x 16 x 1,088 floats of LDS
-// at R = 16, one workgroup of 16 waves per CU).
-template <int R, int JT, int NB, int SMAX, bool DS, int DEPTH, bool PIPE>
+// 16-row tiles: batches of 16 merged entries, windows of TB + 64 bins.
+// R = 8 (512-bin blocks): 2 x 16 x 576 floats of LDS and 6.5 KiB of batch
+// metadata, two workgroups of 16 waves per CU; R = 16 (1,024-bin blocks):
+// 2 x 16 x 1,088 floats, one workgroup per CU.  Measured on the street
+// canyon (T = 6,000, 4 x 4 blocks, 1.71 M merged entries): R = 8 98 ms,
+// R = 16 108 ms; with the metadata broadcast by v_readlane from per-lane
+// loads instead of staged in LDS, and the windows staged through
+// registers, R = 16 took 131 ms.  The diffuse + specular plan runs R = 8
+// with one workgroup per CU (its second sums need 95 VGPRs).
+template <int R, int JT, int SMAX, bool DS>
 int launch_tile_lds(int64_t P, int64_t B, int64_t T, int64_t M,
                     const int64_t *tptr, const int64_t *grp_ptr,
-                    const int64_t *gdiff, const int32_t *mrow,
+                    const int64_t *gdiff, const int32_t *mspan,
                     const int32_t *mlag, const double *mw, const double *mdw,
                     const float *prev, float *gsum, hipStream_t s) {
   const int64_t n_tiles = ceil_div(P, JT);
@@ -715,13 +717,31 @@ int launch_tile_lds(int64_t P, int64_t B, int64_t T, int64_t M,
   const int64_t nb = n_tiles * B * nt;
   SPR_REQUIRE(nb < (int64_t)0x7fffffff, "exchange grid too large");
   SPR_REQUIRE(T < ((int64_t)1 << 29), "histogram too long (T >= 2^29)");
-  hipLaunchKernelGGL(
-      (tile_lds_gather_kernel<R, JT, NB, SMAX, DS, DEPTH, PIPE>),
+  hipLaunchKernelGGL((tile_lds_gather_kernel<R, JT, SMAX, DS>),
                      dim3((unsigned)nb), dim3(64 * JT), 0, s, P, (int)B,
-                     (int)T, (int)nt, nb, M, tptr, grp_ptr, gdiff, mrow,
-                     mlag, mw, mdw, prev, gsum);
+                     (int)T, (int)nt, nb, M, tptr, grp_ptr, gdiff,
+                     reinterpret_cast<const int4 *>(mspan), mlag, mw, mdw,
+                     prev, gsum);
   SPR_LAUNCH_CHECK();
   return SPR_OK;
+}
+
+// mspan[m] = {source row, largest and smallest delay of the merged entry's
+// present slots, 0}.
+__global__ __launch_bounds__(256) void tile_span_kernel(
+    int64_t M, int JT, const int32_t *__restrict__ mrow,
+    const int32_t *__restrict__ mlag, int4 *__restrict__ mspan) {
+  const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= M) return;
+  int32_t mx = 0, mn = kLagMask;
+  for (int kk = 0; kk < JT; ++kk) {
+    const int32_t l = mlag[m * JT + kk] & kLagMask;
+    if (l != kLagMask) {
+      mx = max(mx, l);
+      mn = min(mn, l);
+    }
+  }
+  mspan[m] = make_int4(mrow[m], mx, mn == kLagMask ? mx : mn, 0);
 }
 
 unsigned grid_of(int64_t n) { return (unsigned)ceil_div(n > 0 ? n : 1, 256); }
@@ -802,6 +822,21 @@ extern "C" int spr_plan_tile_fill(int64_t P, int64_t JT, int64_t B,
   return SPR_OK;
 }
 
+extern "C" int spr_plan_tile_span(int64_t M, int64_t JT, const int32_t *mrow,
+                                  const int32_t *mlag, int32_t *mspan,
+                                  void *stream) {
+  clear_error();
+  SPR_REQUIRE(M >= 0 && JT >= 2 && JT <= 16,
+              "spr_plan_tile_span: bad sizes (JT must be 2..16)");
+  if (M == 0) return SPR_OK;
+  SPR_REQUIRE(mrow && mlag && mspan, "spr_plan_tile_span: null pointer");
+  hipLaunchKernelGGL(tile_span_kernel, dim3(grid_of(M)), dim3(256), 0,
+                     as_stream(stream), M, (int)JT, mrow, mlag,
+                     reinterpret_cast<int4 *>(mspan));
+  SPR_LAUNCH_CHECK();
+  return SPR_OK;
+}
+
 extern "C" int spr_exchange_gather_tiled(int64_t P, int64_t JT, int64_t B,
                                          int64_t T, int64_t M,
                                          const int64_t *tptr,
@@ -810,9 +845,9 @@ extern "C" int spr_exchange_gather_tiled(int64_t P, int64_t JT, int64_t B,
                                          const int32_t *mlag,
                                          const double *mw, const float *prev,
                                          float *gsum, void *stream) {
-  SPR_REQUIRE(P > 0 && B > 0 && T > 0 && M >= 0 &&
-                  (JT == 4 || JT == 8 || JT == 16),
-              "spr_exchange_gather_tiled: bad sizes (JT must be 4, 8 or 16)");
+  SPR_REQUIRE(P > 0 && B > 0 && T > 0 && M >= 0 && (JT == 4 || JT == 8),
+              "spr_exchange_gather_tiled: bad sizes (JT must be 4 or 8; "
+              "16-row tiles: spr_exchange_gather_tiled_shape)");
   SPR_REQUIRE(tptr && grp_ptr && mrow && mlag && mw && prev && gsum,
               "spr_exchange_gather_tiled: null pointer");
   return launch_tile_gather(P, (int)JT, B, T, M, tptr, grp_ptr, mrow, mlag,
@@ -822,44 +857,34 @@ extern "C" int spr_exchange_gather_tiled(int64_t P, int64_t JT, int64_t B,
 extern "C" int spr_exchange_gather_tiled_ds(
     int64_t P, int64_t JT, int64_t B, int64_t T, int64_t M,
     const int64_t *tptr, const int64_t *gstart, const int64_t *gdiff,
-    const int32_t *mrow, const int32_t *mlag, const double *mw,
+    const int32_t *mspan, const int32_t *mlag, const double *mw,
     const double *mdw, const float *prev, float *gsum, void *stream) {
   SPR_REQUIRE(P > 0 && B > 0 && T > 0 && M >= 0 && JT == 16,
               "spr_exchange_gather_tiled_ds: bad sizes (JT must be 16)");
-  SPR_REQUIRE(tptr && gstart && gdiff && mrow && mlag && mw && mdw && prev &&
+  SPR_REQUIRE(tptr && gstart && gdiff && mspan && mlag && mw && mdw && prev &&
                   gsum,
               "spr_exchange_gather_tiled_ds: null pointer");
-  return launch_tile_lds<8, 16, 16, 64, true, 1, false>(
-      P, B, T, M, tptr, gstart, gdiff, mrow, mlag, mw, mdw, prev, gsum,
-      as_stream(stream));
+  return launch_tile_lds<8, 16, 64, true>(P, B, T, M, tptr, gstart, gdiff,
+                                          mspan, mlag, mw, mdw, prev, gsum,
+                                          as_stream(stream));
 }
 
 extern "C" int spr_exchange_gather_tiled_shape(
     int64_t P, int64_t JT, int64_t R, int64_t B, int64_t T, int64_t M,
-    const int64_t *tptr, const int64_t *grp_ptr, const int32_t *mrow,
+    const int64_t *tptr, const int64_t *grp_ptr, const int32_t *mspan,
     const int32_t *mlag, const double *mw, const float *prev, float *gsum,
     void *stream) {
   SPR_REQUIRE(P > 0 && B > 0 && T > 0 && M >= 0 && JT == 16 &&
-                  (R == 8 || R == 16 || R == -8 || R == -16),
-              "spr_exchange_gather_tiled_shape: bad sizes (JT 16; R 16, 8, "
-              "-8: 8 with two batches in flight, -16: 16 pipelined over "
-              "the present entries)");
-  SPR_REQUIRE(tptr && grp_ptr && mrow && mlag && mw && prev && gsum,
+                  (R == 8 || R == 16),
+              "spr_exchange_gather_tiled_shape: bad sizes (JT 16, R 16 or 8)");
+  SPR_REQUIRE(tptr && grp_ptr && mspan && mlag && mw && prev && gsum,
               "spr_exchange_gather_tiled_shape: null pointer");
   hipStream_t s = as_stream(stream);
   if (R == 8)
-    return launch_tile_lds<8, 16, 16, 64, false, 1, false>(
-        P, B, T, M, tptr, grp_ptr, nullptr, mrow, mlag, mw, nullptr, prev,
-        gsum, s);
-  if (R == -16)
-    return launch_tile_lds<16, 16, 16, 64, false, 1, true>(
-        P, B, T, M, tptr, grp_ptr, nullptr, mrow, mlag, mw, nullptr, prev,
-        gsum, s);
-  if (R == -8)
-    return launch_tile_lds<8, 16, 16, 64, false, 2, false>(
-        P, B, T, M, tptr, grp_ptr, nullptr, mrow, mlag, mw, nullptr, prev,
-        gsum, s);
-  return launch_tile_lds<16, 16, 16, 64, false, 1, false>(
-      P, B, T, M, tptr, grp_ptr, nullptr, mrow, mlag, mw, nullptr, prev,
-      gsum, s);
+    return launch_tile_lds<8, 16, 64, false>(P, B, T, M, tptr, grp_ptr,
+                                             nullptr, mspan, mlag, mw,
+                                             nullptr, prev, gsum, s);
+  return launch_tile_lds<16, 16, 64, false>(P, B, T, M, tptr, grp_ptr,
+                                            nullptr, mspan, mlag, mw, nullptr,
+                                            prev, gsum, s);
 }

@@ -464,8 +464,11 @@ class TilePlan:
     # diffuse group (-1: none)
     mdw: torch.Tensor = None
     gdiff: torch.Tensor = None
-    # 16-row tiles: bins per lane of the gather's workgroups (16 or 8)
+    # 16-row tiles: bins per lane of the gather's workgroups (16 or 8) and
+    # each merged entry's span [M, 4] (int32: source row, largest and
+    # smallest delay of its slots, 0), from which the gather stages windows
     R: int = 16
+    mspan: torch.Tensor = None
 
 
 def _rows_in_order(plan, order):
@@ -590,6 +593,10 @@ def tile_plan(plan, jt=4, min_merge=1.5, order=None):
          ptr(payload), ptr(first), ptr(mrow), ptr(mlag), ptr(mw),
          stream_ptr())
     tp = TilePlan(jt, M, P, gstart.contiguous(), tptr, mrow, mlag, mw)
+    if jt == 16:
+        tp.mspan = torch.empty((M, 4), dtype=torch.int32, device=dev)
+        call("spr_plan_tile_span", M, jt, ptr(mrow), ptr(mlag),
+             ptr(tp.mspan), stream_ptr())
     if ds:
         # the same fill with the diffuse weights (mrow and mlag are
         # written again with the same values)
@@ -839,7 +846,7 @@ class ExchangeState:
         if plan.diffuse_specular and plan.tiles is not None:
             tp = plan.tiles
             call("spr_exchange_gather_tiled_ds", tp.P, tp.JT, B, T, tp.M,
-                 ptr(tp.tptr), ptr(tp.gstart), ptr(tp.gdiff), ptr(tp.mrow),
+                 ptr(tp.tptr), ptr(tp.gstart), ptr(tp.gdiff), ptr(tp.mspan),
                  ptr(tp.mlag), ptr(tp.mw), ptr(tp.mdw), ptr(self.prev),
                  ptr(self._gsum(plan)), stream_ptr())
             return
@@ -854,7 +861,7 @@ class ExchangeState:
             if tp.JT == 16:
                 call("spr_exchange_gather_tiled_shape", tp.P, tp.JT, tp.R,
                      B, T, tp.M, ptr(tp.tptr), ptr(tp.gstart),
-                     ptr(tp.mrow), ptr(tp.mlag), ptr(tp.mw), ptr(self.prev),
+                     ptr(tp.mspan), ptr(tp.mlag), ptr(tp.mw), ptr(self.prev),
                      ptr(self._gsum(plan)), stream_ptr())
                 return
             call("spr_exchange_gather_tiled", tp.P, tp.JT, B, T, tp.M,

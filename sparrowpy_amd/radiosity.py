@@ -645,31 +645,40 @@ class DirectionalRadiosityFast:
     # Use the grouped plan when groups are at most this fraction of pairs
     # (each group costs D FMAs per bin, each pair one FMA).
     GROUPED_MAX_GROUP_FRACTION = 0.25
-    # Rows per destination tile of the grouped plan's gather (4 or 8; 0
+    # Rows per destination tile of the grouped plan's gather (4, 8 or 16; 0
     # gathers row by row); tiles are used when they merge at least
-    # TILE_MIN_MERGE entries into one.
-    TILE_ROWS = 4
+    # TILE_MIN_MERGE entries into one.  16-row tiles stage each merged
+    # source row in LDS once for all 16 rows (one wave per row).
+    TILE_ROWS = 16
     TILE_MIN_MERGE = 1.5
     # Tiles as (a, b) blocks of neighbouring patches of one wall instead of
     # TILE_ROWS consecutive patch ids (a * b must equal TILE_ROWS); None:
-    # consecutive ids.
-    TILE_BLOCK = None
-    # 16-row tiles: bins per lane of the gather (16; 8; -8: 8 with two
-    # batches of loads in flight)
-    TILE_R = 16
+    # consecutive ids; "auto": 4 x 4 blocks for 16 rows, 2 x 4 for 8, 2 x 2
+    # for 4.
+    TILE_BLOCK = "auto"
+    # 16-row tiles: bins per lane of the gather (8: two workgroups per CU;
+    # or 16)
+    TILE_R = 8
     # tile blocks padded to whole blocks at wall edges (empty slots), or
     # run together
     TILE_PAD = True
 
+    _AUTO_BLOCKS = {16: (4, 4), 8: (2, 4), 4: (2, 2)}
+
     def _tile_order(self):
         """Destination order of the tiles (None: consecutive patch ids)."""
-        if self.TILE_BLOCK is None:
+        blk = self.TILE_BLOCK
+        if isinstance(blk, str):
+            if blk != "auto":
+                raise ValueError('TILE_BLOCK must be None, "auto" or (a, b)')
+            blk = self._AUTO_BLOCKS.get(self.TILE_ROWS)
+        if blk is None:
             return None
-        a, bb = self.TILE_BLOCK
+        a, bb = blk
         if a * bb != self.TILE_ROWS:
             raise ValueError("TILE_BLOCK must hold TILE_ROWS patches")
         return ops.block_order(self._patches_points,
-                               self._patch_to_wall_ids, self.TILE_BLOCK,
+                               self._patch_to_wall_ids, (a, bb),
                                pad=self.TILE_PAD)
 
     def _exchange_plan(self, speed_of_sound, time_resolution, kind=None):

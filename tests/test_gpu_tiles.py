@@ -90,6 +90,19 @@ def test_tile_plan_contents(ops, name, jt):
             seen = np.cumsum(present)[(words & flag) != 0]
             ends = gs[gptr[j] + 1:gptr[j + 1] + 1] - rp[j]
             np.testing.assert_array_equal(seen, ends)
+    if jt == 16:
+        # each merged entry's span: its row and its slots' delay range
+        span = _np(tp.mspan)
+        assert span.shape == (tp.M, 4)
+        lags = np.where((mlag & LAG_MASK) != LAG_MASK, mlag & LAG_MASK,
+                        -1).astype(np.int64)
+        np.testing.assert_array_equal(span[:, 0], mrow)
+        np.testing.assert_array_equal(span[:, 1], lags.max(axis=1))
+        big = np.where(lags >= 0, lags, np.iinfo(np.int64).max)
+        np.testing.assert_array_equal(span[:, 2], big.min(axis=1))
+        assert (span[:, 3] == 0).all()
+    else:
+        assert tp.mspan is None
 
 
 def _random_state(ops, P, D, B, T, seed):
@@ -147,8 +160,7 @@ def test_tiled_gather_bitexact_wide_delays(ops, jt):
 
 
 @pytest.mark.parametrize("jt,block,R,pad", [
-    (16, (4, 4), 16, True), (16, (4, 4), 8, True), (16, (4, 4), -8, True),
-    (16, None, -8, True), (16, (4, 4), -16, True), (16, None, -16, True),
+    (16, (4, 4), 16, True), (16, (4, 4), 8, True), (16, None, 8, True),
     (16, (4, 4), 16, False), (8, (2, 4), 16, True), (4, (2, 2), 16, True)])
 def test_block_tiles_bitexact(ops, jt, block, R, pad):
     """Tiles as blocks of neighbouring patches (rows permuted, walls'
