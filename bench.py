@@ -26,6 +26,7 @@ import time
 import numpy as np
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, published
+FP64_PEAK_TFLOPS = 78.6  # MI355X FP64 vector (and dense MFMA), published
 
 
 def _scene(name):
@@ -393,9 +394,10 @@ def bench_batched(args, log):
     if args.batched_part in ("both", "batched"):
         be = BatchedExchange(
             items, c, dt, dur, K, tile_rows=args.batch_tile_rows,
-            tile_block=(None if args.batch_tile_block is None else tuple(
-                int(v) for v in args.batch_tile_block.split("x"))),
-            tile_r=args.tile_r if args.tile_r is not None else 16)
+            tile_block=(None if args.batch_tile_block == "none" else
+                        "auto" if args.batch_tile_block == "auto" else tuple(
+                            int(v) for v in args.batch_tile_block.split("x"))),
+            tile_r=args.tile_r if args.tile_r is not None else 8)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         out = be.run()
@@ -451,6 +453,17 @@ def bench_batched(args, log):
     }
 
 
+def _kernel_name(full):
+    """Kernel name with its template arguments, without return type,
+    namespaces or parameter list, as rocprofv3 prints it demangled:
+    "void ns::(anonymous namespace)::k<8, 16>(long, ...)" -> "k<8, 16>"."""
+    s = full.replace("(anonymous namespace)::", "").split("(")[0].strip()
+    if s.startswith("void "):
+        s = s[len("void "):]
+    head, sep, tail = s.partition("<")
+    return head.split("::")[-1] + sep + tail
+
+
 def _counter_means(root, counter):
     """{kernel name: mean value of ``counter`` per dispatch} from the
     rocprofv3 CSV output under ``root`` (``*counter_collection.csv``)."""
@@ -466,7 +479,7 @@ def _counter_means(root, counter):
             for row in csv.DictReader(f):
                 if row.get("Counter_Name") != counter:
                     continue
-                name = row["Kernel_Name"].split("(")[0].strip()
+                name = _kernel_name(row["Kernel_Name"])
                 key = (name, row.get("Dispatch_Id", len(vals)))
                 vals[key] = vals.get(key, 0.0) + float(row["Counter_Value"])
     per = {}
@@ -534,14 +547,14 @@ def main(argv=None):
                     choices=["both", "batched", "loop"],
                     help="--config batched: run both paths, or only one "
                          "(to profile it alone)")
-    ap.add_argument("--batch-tile-rows", type=int, default=4,
+    ap.add_argument("--batch-tile-rows", type=int, default=16,
                     choices=[0, 4, 8, 16],
                     help="--config batched: destination rows per tile of "
                          "the batched classes' gathers")
-    ap.add_argument("--batch-tile-block", default=None,
+    ap.add_argument("--batch-tile-block", default="auto",
                     help="--config batched: tiles as AxB blocks of each "
                          "room's patches (e.g. 4x4 with --batch-tile-rows "
-                         "16)")
+                         "16), none, or auto (the class default)")
     ap.add_argument("--no-adjoint", action="store_true",
                     help="skip the adjoint/forward timing")
     ap.add_argument("--plan", default="auto",
@@ -776,6 +789,18 @@ def main(argv=None):
         },
         "setup": t,
     }
+    if dom in ("tile_lds_gather_kernel", "tile_gather_kernel",
+               "group_gather_kernel"):
+        # the gather of the grouped plan: one FP64 FMA per (entry, band,
+        # bin), i.e. per update, plus one f32 -> f64 conversion of the
+        # source bin on the same vector ALU
+        f64 = 2.0 * U
+        line["roofline"]["fp64_vector"] = {
+            "flops": f64, "achieved_tflops": f64 / dom_s / 1e12,
+            "peak_tflops": FP64_PEAK_TFLOPS,
+            "frac": f64 / dom_s / 1e12 / FP64_PEAK_TFLOPS,
+            "model": "one FP64 FMA per update (each with an f32->f64 "
+                     "conversion on the same VALU)"}
     traffic, why = measured_traffic(dom, line["config"])
     if traffic is not None:
         line["roofline"]["traffic"] = traffic

@@ -26,6 +26,7 @@ import numpy as np
 import torch
 
 from . import ops
+from .radiosity import DirectionalRadiosityFast
 
 __all__ = ["BatchItem", "BatchedExchange", "run_per_instance"]
 
@@ -81,6 +82,10 @@ def _concat_plans(plans, offsets):
             p0.rtab if p0.diffuse_specular else _cat([p.rtab for p in plans]),
             _cat([p.dscal for p in plans], 1) if p0.diffuse_specular
             else None)
+        if p0.diffuse_specular:
+            # each entry's specular direction (D: none), the merge key of
+            # the plan's 16-row tiles
+            out.entry_key = _cat([p.entry_key for p in plans])
         return out
     return ops.ExchangePlan(
         P, D, p0.B, row_ptr,
@@ -103,10 +108,21 @@ class BatchedExchange:
 
     def __init__(self, items, speed_of_sound, time_resolution, duration,
                  max_order, autotune=True, memory_budget=None, graphs=True,
-                 tile_rows=4, tile_block=None, tile_r=16):
+                 tile_rows=16, tile_block="auto", tile_r=8):
         self.items = list(items)
         self.tile_rows = int(tile_rows)
-        self.tile_block = tile_block   # e.g. (4, 4) with tile_rows 16
+        # tiles as blocks of each room's patches: (a, b) with a * b =
+        # tile_rows, None (consecutive patch ids) or "auto" (the block
+        # DirectionalRadiosityFast uses for tile_rows)
+        if isinstance(tile_block, str):
+            if tile_block != "auto":
+                raise ValueError('tile_block must be None, "auto" or (a, b)')
+            tile_block = DirectionalRadiosityFast._AUTO_BLOCKS.get(
+                self.tile_rows)
+        if tile_block is not None and \
+                tile_block[0] * tile_block[1] != self.tile_rows:
+            raise ValueError("tile_block must hold tile_rows patches")
+        self.tile_block = tile_block
         self.tile_r = int(tile_r)
         self.c, self.dt = float(speed_of_sound), float(time_resolution)
         self.T = int(duration / time_resolution)

@@ -53,11 +53,15 @@ def _np(d):
     return {k: v.cpu().numpy() for k, v in d.items()}
 
 
-@pytest.mark.parametrize("graphs", [True, False])
-def test_batched_matches_per_instance(items, graphs):
+@pytest.mark.parametrize("graphs,tiles", [
+    (True, {}), (False, {}),
+    # 4-row tiles of consecutive patches; 16-row tiles at 16 bins per lane
+    (True, dict(tile_rows=4, tile_block=None)),
+    (True, dict(tile_rows=16, tile_block=None, tile_r=16))])
+def test_batched_matches_per_instance(items, graphs, tiles):
     from sparrowpy_amd.batch import BatchedExchange, run_per_instance
     ref = _np(run_per_instance(items, C, DT, DUR, K))
-    be = BatchedExchange(items, C, DT, DUR, K, graphs=graphs)
+    be = BatchedExchange(items, C, DT, DUR, K, graphs=graphs, **tiles)
     out = _np(be.run())
     assert set(out) == set(ref)
     for key in ref:
