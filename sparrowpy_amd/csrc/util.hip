@@ -21,38 +21,111 @@ int fail(int code, const char *fmt, ...) {
 
 void clear_error() { g_last_error[0] = '\0'; }
 
-// In-place exclusive scan of v[0..n-1] by one block of 1024 threads;
-// v[n] receives the total.  Each thread scans a contiguous chunk.
-__global__ __launch_bounds__(1024) void scan_exclusive_kernel(int64_t *v,
-                                                              int64_t n) {
-  __shared__ int64_t part[1024];
-  const int tid = threadIdx.x;
-  const int64_t chunk = (n + 1023) / 1024;
-  const int64_t lo = tid * chunk;
-  const int64_t hi = lo + chunk < n ? lo + chunk : n;
-  int64_t s = 0;
-  for (int64_t k = lo; k < hi; ++k) s += v[k];
-  part[tid] = s;
+// In-place exclusive scan of v[0..n-1], v[n] receiving the total, over
+// the whole device and without workspace.  Pass 1: each workgroup scans
+// its tile of kScanTile values in place (each thread kScanItems
+// consecutive values, the threads' sums scanned in LDS) and parks the
+// tile's total in the tile's first slot, whose local prefix is 0.  Pass 2:
+// one workgroup scans those totals (stride kScanTile) in place into the
+// tiles' offsets and writes the grand total to v[n].  Pass 3: each tile
+// adds its offset to its other values.
+constexpr int kScanThreads = 256;
+constexpr int kScanItems = 16;
+constexpr int64_t kScanTile = (int64_t)kScanThreads * kScanItems;
+
+// Exclusive scan of one value per thread across the workgroup (LDS);
+// returns this thread's prefix, *total the workgroup's sum.
+template <int NT>
+__device__ __forceinline__ int64_t block_exclusive_scan(int64_t x,
+                                                        int64_t *total) {
+  __shared__ int64_t part[NT];
+  const int tid = (int)threadIdx.x;
+  part[tid] = x;
   __syncthreads();
-  // Hillis-Steele inclusive scan of the 1024 partial sums
-  for (int off = 1; off < 1024; off <<= 1) {
-    int64_t add = tid >= off ? part[tid - off] : 0;
+  for (int off = 1; off < NT; off <<= 1) {
+    const int64_t add = tid >= off ? part[tid - off] : 0;
     __syncthreads();
     part[tid] += add;
     __syncthreads();
   }
-  int64_t run = part[tid] - s;  // exclusive prefix of this chunk
+  *total = part[NT - 1];
+  const int64_t incl = part[tid];
+  __syncthreads();  // part is reused by the next call
+  return incl - x;
+}
+
+__global__ __launch_bounds__(kScanThreads) void scan_tiles_kernel(int64_t *v,
+                                                                 int64_t n) {
+  const int64_t base = (int64_t)blockIdx.x * kScanTile;
+  const int64_t lo = base + (int64_t)threadIdx.x * kScanItems;
+  int64_t x[kScanItems];
+  int64_t s = 0;
+#pragma unroll
+  for (int i = 0; i < kScanItems; ++i) {
+    x[i] = lo + i < n ? v[lo + i] : 0;
+    s += x[i];
+  }
+  int64_t total;
+  int64_t run = block_exclusive_scan<kScanThreads>(s, &total);
+#pragma unroll
+  for (int i = 0; i < kScanItems; ++i) {
+    // the tile's first slot gets the tile's total (its prefix is 0)
+    if (lo + i < n) v[lo + i] = lo + i == base ? total : run;
+    run += x[i];
+  }
+}
+
+__global__ __launch_bounds__(1024) void scan_totals_kernel(int64_t *v,
+                                                           int64_t n,
+                                                           int64_t ntiles) {
+  // ntiles totals at stride kScanTile; each thread a contiguous run of
+  // them (few: one per 4,096 values)
+  const int tid = (int)threadIdx.x;
+  const int64_t chunk = (ntiles + 1023) / 1024;
+  const int64_t lo = tid * chunk;
+  const int64_t hi = lo + chunk < ntiles ? lo + chunk : ntiles;
+  int64_t s = 0;
+  for (int64_t k = lo; k < hi; ++k) s += v[k * kScanTile];
+  int64_t total;
+  int64_t run = block_exclusive_scan<1024>(s, &total);
   for (int64_t k = lo; k < hi; ++k) {
-    int64_t c = v[k];
-    v[k] = run;
+    const int64_t c = v[k * kScanTile];
+    v[k * kScanTile] = run;
     run += c;
   }
-  if (tid == 1023) v[n] = part[1023];
+  if (tid == 0) v[n] = total;
+}
+
+__global__ __launch_bounds__(kScanThreads) void scan_add_kernel(int64_t *v,
+                                                               int64_t n) {
+  const int64_t base = (int64_t)blockIdx.x * kScanTile;
+  const int64_t off = v[base];  // the tile's offset (pass 2)
+  const int64_t lo = base + (int64_t)threadIdx.x * kScanItems;
+#pragma unroll
+  for (int i = 0; i < kScanItems; ++i)
+    if (lo + i < n && lo + i != base) v[lo + i] += off;
 }
 
 int scan_exclusive(int64_t *v, int64_t n, hipStream_t s) {
-  hipLaunchKernelGGL(scan_exclusive_kernel, dim3(1), dim3(1024), 0, s, v, n);
+  if (n <= 0) {
+    hipLaunchKernelGGL(scan_totals_kernel, dim3(1), dim3(1024), 0, s, v,
+                       (int64_t)0, (int64_t)0);
+    SPR_LAUNCH_CHECK();
+    return SPR_OK;
+  }
+  const int64_t ntiles = ceil_div(n, kScanTile);
+  SPR_REQUIRE(ntiles < (int64_t)0x7fffffff, "scan too long");
+  hipLaunchKernelGGL(scan_tiles_kernel, dim3((unsigned)ntiles),
+                     dim3(kScanThreads), 0, s, v, n);
   SPR_LAUNCH_CHECK();
+  hipLaunchKernelGGL(scan_totals_kernel, dim3(1), dim3(1024), 0, s, v, n,
+                     ntiles);
+  SPR_LAUNCH_CHECK();
+  if (ntiles > 1) {
+    hipLaunchKernelGGL(scan_add_kernel, dim3((unsigned)ntiles),
+                       dim3(kScanThreads), 0, s, v, n);
+    SPR_LAUNCH_CHECK();
+  }
   return SPR_OK;
 }
 
