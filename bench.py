@@ -29,10 +29,10 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, published
 FP64_PEAK_TFLOPS = 78.6  # MI355X FP64 vector (and dense MFMA), published
 
 
-def _scene(name):
+def _scene(name, brdf="scattering"):
     from sparrowpy_amd import scenes
     if name == "canyon":
-        return scenes.street_canyon()
+        return scenes.street_canyon(brdf=brdf)
     if name == "shoebox":
         return scenes.directional_shoebox()
     if name == "small":
@@ -557,6 +557,12 @@ def main(argv=None):
                          "16), none, or auto (the class default)")
     ap.add_argument("--no-adjoint", action="store_true",
                     help="skip the adjoint/forward timing")
+    ap.add_argument("--brdf", default="scattering",
+                    choices=["scattering", "directional"],
+                    help="--config canyon: the walls' BRDFs from scattering "
+                         "coefficients (constant + specular rows) or from "
+                         "seeded directional scattering coefficients "
+                         "(general rows, shared per wall)")
     ap.add_argument("--plan", default="auto",
                     choices=["auto", "ds", "grouped", "generic"],
                     help="exchange plan (auto: grouped when sources share "
@@ -585,7 +591,7 @@ def main(argv=None):
         return copy_calibration()
     if args.traffic_from:
         from sparrowpy_amd import geometry
-        sc = _scene(args.config)
+        sc = _scene(args.config, args.brdf)
         wp = np.array([w.pts for w in sc.walls])
         wn = np.array([w.normal for w in sc.walls])
         n_patches = int(geometry.process_patches(wp, wn,
@@ -613,7 +619,7 @@ def main(argv=None):
         fn = bench_batched if args.config == "batched" else bench_inverse
         print(json.dumps(fn(args, blog)), flush=True)
         return 0
-    sc = _scene(args.config)
+    sc = _scene(args.config, args.brdf)
     torch.cuda.init()
     dev = torch.device("cuda", 0)
     t = {}

@@ -12,7 +12,7 @@ import dataclasses
 
 import numpy as np
 
-from .brdf import create_from_scattering
+from .brdf import create_from_directional_scattering, create_from_scattering
 from .coordinates import Coordinates, FrequencyData
 from .geometry import Polygon
 from .samplings import hemisphere, sph_gaussian
@@ -151,14 +151,23 @@ def _facade_rects(x_len, z_len, holes):
 def street_canyon(seed=3, length=100., width=20., height=15.,
                   n_recesses=10, recess_depth=1.0, patch_size=1.0,
                   n_dirs=(4, 8), n_bands=8, duration=1.5, rate=4000.,
-                  max_order=40, n_receivers=16):
+                  max_order=40, n_receivers=16, brdf="scattering"):
     """Open street canyon: 100x20 m ground and two 100x15 m facades at 1 m
     patches, facades broken by 10 seeded rectangular recesses 2-4 m wide and
     high and ``recess_depth`` deep (each adds a back wall and four reveal
     walls and produces occlusion).  32 BRDF directions (4 colatitudes x 8
     azimuths), 8 octave bands, per-wall alpha in [0.05, 0.3] and s in
     [0.1, 0.9], 2 sources at 1.5 m height, receivers along the canyon axis,
-    4 kHz bins over 1.5 s, order 40."""
+    4 kHz bins over 1.5 s, order 40.
+
+    ``brdf="directional"``: every wall's BRDF from a seeded directional
+    scattering coefficient (``create_from_directional_scattering``, each
+    incoming direction's row uniform in [0.2, 1] and normalised to sum to
+    1 over the outgoing directions, per band) with the wall's absorption:
+    rows that are neither a constant plus one specular entry nor shared
+    between walls."""
+    if brdf not in ("scattering", "directional"):
+        raise ValueError('brdf must be "scattering" or "directional"')
     rng = np.random.default_rng(seed)
     L, Wd, H, dep = length, width, height, recess_depth
     holes = {0: [], 1: []}
@@ -203,16 +212,26 @@ def street_canyon(seed=3, length=100., width=20., height=15.,
     dirs = hemisphere(sph_gaussian(n_points=(2 * n_dirs[0], n_dirs[1])))
     freqs = OCTAVE_BANDS[:n_bands]
     groups = []
+    drng = np.random.default_rng(seed + 1000)
     for i in range(len(walls)):
         alpha = rng.uniform(0.05, 0.3, n_bands)
         s = rng.uniform(0.1, 0.9, n_bands)
-        groups.append(([i], _brdf(dirs, s, alpha, freqs)))
+        if brdf == "scattering":
+            groups.append(([i], _brdf(dirs, s, alpha, freqs)))
+            continue
+        n = dirs.csize
+        sd = drng.uniform(0.2, 1.0, (n, n, n_bands))
+        sd /= sd.sum(axis=1, keepdims=True)
+        groups.append(([i], create_from_directional_scattering(
+            dirs, dirs, FrequencyData(sd, freqs),
+            FrequencyData(alpha.reshape(1, -1), freqs))))
     src = np.array([[L * 0.25, Wd * 0.35, 1.5], [L * 0.7, Wd * 0.6, 1.5]])
     rx = np.linspace(L * 0.05, L * 0.95, n_receivers)
     recv = np.stack([rx, np.full_like(rx, Wd / 2), np.full_like(rx, 1.5)],
                     axis=-1)
     return Scene(
-        name="street-canyon-100x20x15-p1.0", walls=walls,
+        name="street-canyon-100x20x15-p1.0" + (
+            "-dirscat" if brdf == "directional" else ""), walls=walls,
         patch_size=patch_size, frequencies=freqs, brdf_groups=groups,
         directions=dirs, air_attenuation=air_attenuation_octaves(freqs),
         sources=src, receivers=recv, speed_of_sound=343.,

@@ -7,7 +7,9 @@
   FP64 oracle; two orders run on the plan bench.py uses (grouped, on
   destination tiles), and the second order's histograms of those
   destinations (all directions, all 8 bands, all 6,000 bins) are checked
-  against the oracle's order fed the GPU's first order.  The dense oracle
+  against the oracle's order fed the GPU's first order.  The same again
+  with every wall's BRDF from a seeded directional scattering coefficient
+  (create_from_directional_scattering: general rows).  The dense oracle
   cannot hold this scene (F~ alone is 51 GB), so it runs on the sampled
   pairs (oracle.p2p_visibility_pairs, fft_pairs, exchange_rows; pinned
   against the dense functions in test_oracle_subsets.py).
@@ -202,10 +204,14 @@ def _injection_check(r, o, sc, label):
     _log(f"{label}: injection of {r.n_patches} patches within 1e-9")
 
 
-def test_street_canyon_full_size(torch):
+@pytest.mark.parametrize("brdf", ["scattering", "directional"])
+def test_street_canyon_full_size(torch, brdf):
+    """The headline canyon at full size; with ``brdf="directional"`` every
+    wall's BRDF comes from a directional scattering coefficient (general
+    rows, shared per wall: the grouped plan still applies)."""
     from sparrowpy_amd import ops
-    sc = scenes.street_canyon()
-    _log("street canyon: bake")
+    sc = scenes.street_canyon(brdf=brdf)
+    _log(f"street canyon ({brdf} BRDFs): bake")
     r = gpu_scene(sc)
     o = oracle_scene(sc)
     assert r.n_patches == 5122
