@@ -359,6 +359,14 @@ def bench_inverse(args, log):
     }
 
 
+def _tiling_str(t):
+    rows, block, r = t
+    if not rows:
+        return "row by row"
+    blk = "consecutive" if block is None else f"{block[0]}x{block[1]} blocks"
+    return f"{rows} rows, {blk}, {64 * r}-bin blocks"
+
+
 def bench_batched(args, log):
     """Many rooms and sources: the per-instance loop of the reference API
     against the batched, graph-captured exchange on the same baked scenes.
@@ -403,7 +411,8 @@ def bench_batched(args, log):
         out = be.run()
         torch.cuda.synchronize()
         batched_s = time.perf_counter() - t0
-        log(f"batched {batched_s:.2f} s, classes {be.choice}")
+        log(f"batched {batched_s:.2f} s, classes {be.choice}, tiling "
+            f"{_tiling_str(be.tiling)} of {len(be.tiling_times)} timed")
         be.pool = None
         torch.cuda.empty_cache()
     if args.batched_part in ("both", "loop"):
@@ -443,7 +452,11 @@ def bench_batched(args, log):
                                       for it in items)),
                    "updates": U,
                    "plan_per_class": {f"D{k[0]}xB{k[1]}": v
-                                      for k, v in be.choice.items()}},
+                                      for k, v in be.choice.items()},
+                   "tiling": _tiling_str(be.tiling),
+                   "tiling_candidates_ms": {
+                       _tiling_str(t): v * 1e3
+                       for t, v in be.tiling_times.items()}},
         "per_instance_loop": {"s": loop_s, "updates/s": U / loop_s},
         "speedup_vs_loop": loop_s / batched_s,
         "max_rel_err_vs_loop": err, "bake_s": bake_s,

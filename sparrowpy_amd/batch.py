@@ -13,8 +13,10 @@ class are captured once into a HIP graph and replayed; per class the plan
 kind (grouped or diffuse + specular) is picked by timing one order of each
 candidate on the class's largest scene (``autotune``; only that scene's
 plans are built for it, the others once, for the kind chosen); a grouped
-class gathers on destination tiles of ``tile_rows`` rows
-(``ops.tile_plan``).
+class gathers on destination tiles (``ops.tile_plan``), whose shape --
+rows per tile, patch block, bins per lane of the 16-row gather -- is timed
+against the candidates of ``TILINGS`` on the class with the most work and
+the fastest used for every class (``tune_tiles``).
 Classes larger than the memory budget are split into chunks.
 """
 from __future__ import annotations
@@ -106,24 +108,24 @@ class BatchedExchange:
     settings (speed of sound, bin width, duration, order).
     """
 
+    # Destination tilings the autotuner tries per plan kind: (rows per
+    # tile, block of patches, bins per lane of the 16-row gather).
+    TILINGS = {"grouped": [(16, "auto", 8), (16, "auto", 16),
+                           (4, "auto", 8), (0, None, 8)],
+               "ds": [(16, "auto", 8), (0, None, 8)]}
+
     def __init__(self, items, speed_of_sound, time_resolution, duration,
                  max_order, autotune=True, memory_budget=None, graphs=True,
-                 tile_rows=16, tile_block="auto", tile_r=8):
+                 tile_rows=16, tile_block="auto", tile_r=8, tune_tiles=True):
         self.items = list(items)
-        self.tile_rows = int(tile_rows)
         # tiles as blocks of each room's patches: (a, b) with a * b =
         # tile_rows, None (consecutive patch ids) or "auto" (the block
         # DirectionalRadiosityFast uses for tile_rows)
-        if isinstance(tile_block, str):
-            if tile_block != "auto":
-                raise ValueError('tile_block must be None, "auto" or (a, b)')
-            tile_block = DirectionalRadiosityFast._AUTO_BLOCKS.get(
-                self.tile_rows)
-        if tile_block is not None and \
-                tile_block[0] * tile_block[1] != self.tile_rows:
-            raise ValueError("tile_block must hold tile_rows patches")
-        self.tile_block = tile_block
-        self.tile_r = int(tile_r)
+        self.tiling = self._tiling(tile_rows, tile_block, tile_r)
+        # with autotune: the tiling is timed against TILINGS on the class
+        # with the most work and the fastest used for every class
+        self.tune_tiles = bool(tune_tiles)
+        self.tiling_times = {}
         self.c, self.dt = float(speed_of_sound), float(time_resolution)
         self.T = int(duration / time_resolution)
         self.K = int(max_order)
@@ -147,6 +149,24 @@ class BatchedExchange:
                                   idx, g["centers"],
                                   ops.as_dev(r._air_attenuation,
                                              torch.float64))
+
+    @staticmethod
+    def _tiling(rows, block, r):
+        """(rows, block, R) with the block resolved and checked."""
+        rows = int(rows)
+        if isinstance(block, str):
+            if block != "auto":
+                raise ValueError('tile_block must be None, "auto" or (a, b)')
+            block = DirectionalRadiosityFast._AUTO_BLOCKS.get(rows)
+        if block is not None:
+            block = (int(block[0]), int(block[1]))
+            if block[0] * block[1] != rows:
+                raise ValueError("tile_block must hold tile_rows patches")
+        return rows, block, int(r)
+
+    @property
+    def tile_rows(self):
+        return self.tiling[0]
 
     # -- planning ------------------------------------------------------------
     def _plan(self, x, kind):
@@ -199,17 +219,45 @@ class BatchedExchange:
             if not self.autotune:
                 times[k] = len(times)
                 continue
-            # its own buffers: the pool is sized for the chunks
-            run = self._build([(x_big, 0)], {x_big: plan}, pooled=False)
-            st = run["state"]
-            st.step(run["plan"])             # warm-up
-            torch.cuda.synchronize()
-            t0 = time.perf_counter()
-            st.step(run["plan"])
-            torch.cuda.synchronize()
-            times[k] = time.perf_counter() - t0
-            del run, st
+            times[k] = self._time_order(x_big, plan, self.tiling)
         return sorted(times, key=times.get) + ["generic"]
+
+    def _time_order(self, x, plan, tiling):
+        """Seconds of one exchange order of scene x's first instance on
+        ``plan`` with ``tiling`` (after one untimed order)."""
+        # its own buffers: the pool is sized for the chunks
+        run = self._build([(x, 0)], {x: plan}, pooled=False, tiling=tiling)
+        st = run["state"]
+        st.step(run["plan"])             # warm-up
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        st.step(run["plan"])
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        del run, st
+        return dt
+
+    def _tune_tiling(self, classes):
+        """Time the candidate tilings (TILINGS, for the plan kind of the
+        class with the most work) on that class's largest scene; keep the
+        fastest for every class."""
+        def work(xs):
+            return sum(self.items[x].radiosity.n_patches *
+                       self.items[x].radiosity._bake["D"] *
+                       self.items[x].radiosity._bake["n_bins"] *
+                       len(self.items[x].sources) for x in xs)
+        xs, kind, plans = max(classes, key=lambda c: work(c[0]))
+        if kind not in self.TILINGS:
+            return
+        x_big = max(xs, key=lambda x: self.items[x].radiosity.n_patches)
+        cands = [self.tiling]
+        for t in self.TILINGS[kind]:
+            t = self._tiling(*t)
+            if t not in cands:
+                cands.append(t)
+        self.tiling_times = {t: self._time_order(x_big, plans[x_big], t)
+                             for t in cands}
+        self.tiling = min(self.tiling_times, key=self.tiling_times.get)
 
     def _class_plans(self, xs):
         """(kind, {scene: plan}) for the first kind every scene supports."""
@@ -238,30 +286,30 @@ class BatchedExchange:
             G += p.n_groups if isinstance(p, ops.GroupedPlan) else 0
         return self._pool_sizes(PDB, G, B)
 
-    def _build(self, chunk, plans, pooled=True):
+    def _build(self, chunk, plans, pooled=True, tiling=None):
         xs = [x for x, _ in chunk]
         offsets = np.concatenate(
             [[0], np.cumsum([self.items[x].radiosity.n_patches
                              for x in xs])]).astype(np.int64)
         plan = _concat_plans([plans[x] for x in xs], offsets[:-1])
-        if self.tile_rows and isinstance(plan, ops.GroupedPlan) and (
-                not plan.diffuse_specular or self.tile_rows == 16):
+        rows, block, tile_r = tiling or self.tiling
+        if rows and isinstance(plan, ops.GroupedPlan) and (
+                not plan.diffuse_specular or rows == 16):
             # diffuse + specular plans tile at 16 rows only; with a tile
             # block, each room's patches in blocks of that shape, the rooms
             # one after the other
             order = None
-            if self.tile_block is not None:
+            if block is not None:
                 parts = []
                 for x, off in zip(xs, offsets[:-1]):
                     r = self.items[x].radiosity
                     o = ops.block_order(r._patches_points,
-                                        r._patch_to_wall_ids,
-                                        self.tile_block)
+                                        r._patch_to_wall_ids, block)
                     parts.append(np.where(o >= 0, o + off, -1))
                 order = np.concatenate(parts)
-            plan.tiles = ops.tile_plan(plan, self.tile_rows, order=order)
+            plan.tiles = ops.tile_plan(plan, rows, order=order)
             if plan.tiles is not None:
-                plan.tiles.R = self.tile_r
+                plan.tiles.R = tile_r
         e0d, d0 = [], []
         for x, s in chunk:
             r = self.items[x].radiosity
@@ -315,9 +363,14 @@ class BatchedExchange:
         out = {}
         self.updates = 0
         work = []
+        classes = []
         for key, xs in self._classes().items():
             kind, plans = self._class_plans(xs)
             self.choice[key] = kind
+            classes.append((xs, kind, plans))
+        if self.autotune and self.tune_tiles:
+            self._tune_tiling(classes)
+        for xs, kind, plans in classes:
             work.append((plans, self._chunks(xs, plans)))
         # the pool is sized once, for the largest chunk of every class: a
         # freed buffer of tens of GiB is soon pinned by small allocations

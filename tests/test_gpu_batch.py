@@ -56,13 +56,21 @@ def _np(d):
 @pytest.mark.parametrize("graphs,tiles", [
     (True, {}), (False, {}),
     # 4-row tiles of consecutive patches; 16-row tiles at 16 bins per lane
-    (True, dict(tile_rows=4, tile_block=None)),
-    (True, dict(tile_rows=16, tile_block=None, tile_r=16))])
+    (True, dict(tile_rows=4, tile_block=None, tune_tiles=False)),
+    (True, dict(tile_rows=16, tile_block=None, tile_r=16,
+                tune_tiles=False))])
 def test_batched_matches_per_instance(items, graphs, tiles):
     from sparrowpy_amd.batch import BatchedExchange, run_per_instance
     ref = _np(run_per_instance(items, C, DT, DUR, K))
     be = BatchedExchange(items, C, DT, DUR, K, graphs=graphs, **tiles)
     out = _np(be.run())
+    if tiles.get("tune_tiles", True):
+        # the tuner timed every candidate of the tuned class's kind and
+        # kept the fastest
+        assert be.tiling in be.tiling_times
+        assert be.tiling_times[be.tiling] == min(be.tiling_times.values())
+    else:
+        assert not be.tiling_times
     assert set(out) == set(ref)
     for key in ref:
         assert out[key].shape == ref[key].shape
