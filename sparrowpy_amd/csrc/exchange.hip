@@ -225,7 +225,11 @@ __device__ __forceinline__ void store_tile(const double (&acc)[DP][R], int D,
 
 // Generic plan: every entry carries its own coefficient row coef[e, b, :].
 // The next chunk's gathers are issued before the current chunk's FMAs so
-// that C * R loads per lane are in flight while the VALU works.
+// that C * R loads per lane are in flight while the VALU works.  The
+// coefficient rows (each used once, so a scalar load per entry would wait
+// for L2 every time) come through LDS: the workgroup loads the rows of the
+// next NBE entries with coalesced vector loads while the current NBE are
+// summed, and every wave reads the current rows by broadcast.
 template <int DP, int R, int BS, int C>
 __global__ __launch_bounds__(BS) void exchange_step_kernel(
     int D, int B, int T, int n_tblk, int64_t nblocks,
@@ -234,6 +238,11 @@ __global__ __launch_bounds__(BS) void exchange_step_kernel(
     const double *__restrict__ coef, const float *__restrict__ prev,
     float *__restrict__ cur, float *__restrict__ tot_hi,
     float *__restrict__ tot_lo) {
+  constexpr int NBE = 16;                      // entries per row batch
+  constexpr int NW = NBE * DP;                 // doubles per batch
+  constexpr int NLD = (NW + BS - 1) / BS;      // of them per thread
+  static_assert(NBE % C == 0, "chunks tile a batch");
+  __shared__ double cfs[2][NW];
   const int64_t tile = xcd_tile(blockIdx.x, nblocks);
   const int tb = (int)(tile % n_tblk);
   const int64_t jb = tile / n_tblk;
@@ -249,41 +258,77 @@ __global__ __launch_bounds__(BS) void exchange_step_kernel(
     for (int r = 0; r < R; ++r) acc[d][r] = 0.0;
 
   const int64_t e_begin = row_ptr[j], e_end = row_ptr[j + 1];
-  if (e_end > e_begin) {
+  if (e_end > e_begin) {  // uniform: before any barrier
     auto row_of = [&](int64_t ee) {
       return (int64_t)src[ee] * D + dir[ee];
     };
+    // rows of entries eb .. eb+NBE-1 (band b) into registers, then LDS
+    double pre[NLD];
+    auto rows_load = [&](int64_t eb) {
+#pragma unroll
+      for (int k = 0; k < NLD; ++k) {
+        const int i = (int)threadIdx.x + k * BS;
+        const int q = i / DP, d = i % DP;
+        pre[k] = (i < NW && eb + q < e_end)
+                     ? coef[((eb + q) * B + b) * DP + d]
+                     : 0.0;
+      }
+    };
+    auto rows_store = [&](int buf) {
+#pragma unroll
+      for (int k = 0; k < NLD; ++k) {
+        const int i = (int)threadIdx.x + k * BS;
+        if (i < NW) cfs[buf][i] = pre[k];
+      }
+    };
+    rows_load(e_begin);
+    rows_store(0);
+    __syncthreads();
     float buf[C][R];
     int n[C];
     bool act[C];
     gather_chunk<R, BS, C>(e_begin, e_end, B, T, b, t0, t_end, row_of, lag,
                            prev, buf, n, act);
-    for (int64_t e = e_begin; e < e_end; e += C) {
-      float nbuf[C][R];
-      int nn[C];
-      bool nact[C];
-      gather_chunk<R, BS, C>(e + C, e_end, B, T, b, t0, t_end, row_of, lag,
-                             prev, nbuf, nn, nact);
+    int cb = 0;
+    for (int64_t eb = e_begin; eb < e_end; eb += NBE) {
+      const bool more = eb + NBE < e_end;  // uniform
+      if (more) rows_load(eb + NBE);
 #pragma unroll
-      for (int c = 0; c < C; ++c) {
-        if (!act[c]) continue;
-        const double *cf = coef + ((e + c) * B + b) * DP;
-        double ev[R];
+      for (int q0 = 0; q0 < NBE; q0 += C) {
+        const int64_t e = eb + q0;
+        if (e >= e_end) break;  // uniform
+        float nbuf[C][R];
+        int nn[C];
+        bool nact[C];
+        gather_chunk<R, BS, C>(e + C, e_end, B, T, b, t0, t_end, row_of,
+                               lag, prev, nbuf, nn, nact);
 #pragma unroll
-        for (int r = 0; r < R; ++r) ev[r] = (double)buf[c][r];
+        for (int c = 0; c < C; ++c) {
+          if (!act[c]) continue;
+          const double *cf = &cfs[cb][(q0 + c) * DP];
+          double ev[R];
 #pragma unroll
-        for (int d = 0; d < DP; ++d) {
-          const double cd = cf[d];
+          for (int r = 0; r < R; ++r) ev[r] = (double)buf[c][r];
 #pragma unroll
-          for (int r = 0; r < R; ++r) acc[d][r] = fma(cd, ev[r], acc[d][r]);
+          for (int d = 0; d < DP; ++d) {
+            const double cd = cf[d];
+#pragma unroll
+            for (int r = 0; r < R; ++r) acc[d][r] = fma(cd, ev[r], acc[d][r]);
+          }
+        }
+#pragma unroll
+        for (int c = 0; c < C; ++c) {
+          act[c] = nact[c];
+#pragma unroll
+          for (int r = 0; r < R; ++r) buf[c][r] = nbuf[c][r];
         }
       }
-#pragma unroll
-      for (int c = 0; c < C; ++c) {
-        act[c] = nact[c];
-#pragma unroll
-        for (int r = 0; r < R; ++r) buf[c][r] = nbuf[c][r];
-      }
+      // one barrier per batch: the next rows are in the other buffer
+      // before any wave reads them, and every wave has finished reading
+      // this buffer before it is written again (after the next batch)
+      if (more) rows_store(cb ^ 1);
+      __syncthreads();
+      cb ^= 1;
     }
   }
   store_tile<DP, R, BS>(acc, D, B, T, j, b, t0, cur, tot_hi, tot_lo);
