@@ -204,16 +204,26 @@ KERNEL_SOURCES = ["exchange.hip", "tiles.hip", "spr_exchange.hpp",
                   "spr_util.hpp", "spr_common.hpp"]
 
 
+def _code_only(text):
+    """C++ source without comments and with whitespace runs collapsed: the
+    part of a kernel source that decides what the compiler builds."""
+    import re
+    text = re.sub(r"/\*.*?\*/", " ", text, flags=re.S)
+    text = re.sub(r"//[^\n]*", " ", text)
+    return " ".join(text.split())
+
+
 def kernel_sources_sha():
-    """sha256 over the exchange kernels' sources: a counter profile is only
-    used for the kernels it was measured on."""
+    """sha256 over the exchange kernels' sources (their code, comments
+    left out): a counter profile is only used for the kernels it was
+    measured on."""
     import hashlib
     h = hashlib.sha256()
     csrc = os.path.join(os.path.dirname(os.path.abspath(__file__)),
                         "sparrowpy_amd", "csrc")
     for name in KERNEL_SOURCES:
-        with open(os.path.join(csrc, name), "rb") as f:
-            h.update(name.encode() + b"\0" + f.read())
+        with open(os.path.join(csrc, name)) as f:
+            h.update(name.encode() + b"\0" + _code_only(f.read()).encode())
     return h.hexdigest()[:16]
 
 

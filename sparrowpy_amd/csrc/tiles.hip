@@ -375,14 +375,18 @@ int launch_tile_gather(int64_t P, int JT, int64_t B, int64_t T, int64_t M,
 // windows: per (entry, slot) the LDS offset of the slot's view (maxlag -
 // delay, with the delay word's flags) and the weights, per entry its
 // source row and maxlag.  A wave reads its slot's 16 offset words with one
-// LDS load (one per lane) and takes each with v_readlane, and reads each
-// weight with a broadcast LDS load straight into the FMA's operand, so a
-// present entry costs the wave one v_readlane, one address add, R LDS
-// reads, R conversions and R FP64 FMAs.  Two LDS buffers: the global loads
-// of batch q+1 are in flight while batch q is summed from LDS; one barrier
-// per batch.  Each wave sums its slot's entries in merged order, which
-// within a group is the grouped plan's entry order: the per-group sums are
-// bit-identical to spr_exchange_gather_grouped's.  Requires T < 2^29.
+// LDS load (one per lane), walks only the entries its slot has or ends a
+// group at (ballots over those words), takes each word with v_readlane
+// and reads each weight with a broadcast LDS load straight into the FMA's
+// operand, so a present entry costs the wave one v_readlane, one address
+// add, R LDS reads, R conversions and R FP64 FMAs; the next entry's reads
+// are issued before the current entry is summed.  Two LDS buffers: batch
+// q+1's windows are copied from memory into LDS by the loads themselves
+// (no registers) while batch q is summed; each wave waits for its copies
+// (vmcnt) before the one barrier per batch.  Each wave sums its slot's
+// entries in merged order, which within a group is the grouped plan's
+// entry order: the per-group sums are bit-identical to
+// spr_exchange_gather_grouped's.  Requires T < 2^29.
 typedef __attribute__((address_space(3))) void lds_void_t;
 
 // One buffer load of a dword per lane into LDS: lane l's dword goes to

@@ -118,3 +118,17 @@ def test_profile_kernel_names():
     assert k("spr::(anonymous namespace)::init_kernel(long, float*)") == \
         "init_kernel"
     assert k("__amd_rocclr_copyBuffer") == "__amd_rocclr_copyBuffer"
+
+
+def test_profile_stamp_ignores_comments():
+    """The counter profile is stamped with a hash of the kernels' code:
+    comments and layout do not change it, code does."""
+    if ROOT not in sys.path:
+        sys.path.insert(0, ROOT)
+    import bench
+    c = bench._code_only
+    a = "int f(int x) {  // twice\n  return 2 * x; /* doubled */\n}\n"
+    b = "/* header */\nint f(int x) {\n    return 2 * x;\n}"
+    assert c(a) == c(b) == "int f(int x) { return 2 * x; }"
+    assert c(a) != c(a.replace("2 * x", "3 * x"))
+    assert len(bench.kernel_sources_sha()) == 16
