@@ -170,11 +170,17 @@ class BatchedExchange:
 
     # -- planning ------------------------------------------------------------
     def _plan(self, x, kind):
+        """Scene x's plan of ``kind``, without destination tiles: the
+        batch tiles the concatenation of its instances' plans."""
         r = self.items[x].radiosity
+        rows = r.TILE_ROWS
+        r.TILE_ROWS = 0
         try:
             return r._exchange_plan(self.c, self.dt, kind)
         except ValueError:
             return None
+        finally:
+            r.TILE_ROWS = rows
 
     def _classes(self):
         cls = {}
