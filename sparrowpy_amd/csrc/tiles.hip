@@ -380,12 +380,12 @@ int launch_tile_gather(int64_t P, int JT, int64_t B, int64_t T, int64_t M,
 // and reads each weight with a broadcast LDS load straight into the FMA's
 // operand, so a present entry costs the wave one v_readlane, one address
 // add, R LDS reads, R conversions and R FP64 FMAs; the next entry's reads
-// are issued before the current entry is summed.  Two LDS buffers: batch
-// q+1's windows are copied from memory into LDS by the loads themselves
-// (no registers) while batch q is summed; each wave waits for its copies
-// (vmcnt) before the one barrier per batch.  Each wave sums its slot's
-// entries in merged order, which within a group is the grouped plan's
-// entry order: the per-group sums are bit-identical to
+// are issued before the current entry is summed.  Two LDS buffers (two
+// LDS objects): batch q+1's windows are copied from memory into LDS by the
+// loads themselves (no registers) while batch q is summed; each wave waits
+// for its copies (vmcnt) only before the one barrier per batch.  Each wave
+// sums its slot's entries in merged order, which within a group is the
+// grouped plan's entry order: the per-group sums are bit-identical to
 // spr_exchange_gather_grouped's.  Requires T < 2^29.
 typedef __attribute__((address_space(3))) void lds_void_t;
 
@@ -451,8 +451,15 @@ tile_lds_gather_kernel(
                     (DS ? 3 : 2) * NM <= 64 * JT && NB <= 32 &&
                     (NB & (NB - 1)) == 0,
                 "shape: whole waves stage the metadata");
-  __shared__ float lds[2][NB][W];
+  // the two window buffers are two LDS objects, not one [2][NB][W] array:
+  // the compiler waits (vmcnt) for copies into LDS before any LDS read it
+  // cannot prove to be of another object, and a read of one buffer would
+  // otherwise wait for the other buffer's copies (the next batch's) before
+  // the sums could start
+  __shared__ float lds_a[NB][W];
+  __shared__ float lds_b[NB][W];
   __shared__ BatchMeta<NB, JT, DS> meta[2];
+  auto win = [&](int buf) -> float(*)[W] { return buf ? lds_b : lds_a; };
   int64_t tile;
   int b, tb;
   dest_major_tile(xcd_tile(blockIdx.x, nblocks), nblocks / (B * n_tblk),
@@ -497,7 +504,7 @@ tile_lds_gather_kernel(
     const int need = min(W, TB + (sp.y - sp.z));
     const float *row = prev + (int64_t)sp.x * bstride + (int64_t)b * T;
     const int w0 = t0 - sp.y;
-    float *dst = &lds[buf][k][0];
+    float *dst = &win(buf)[k][0];
     if (w0 >= 0) {
       const __amdgpu_buffer_rsrc_t rs = make_rsrc(
           row + w0, w0 < T ? (uint32_t)(T - w0) * 4u : 0u);
@@ -573,7 +580,7 @@ tile_lds_gather_kernel(
   auto read_entry = [&](int buf, int idx, int32_t words, Cur &c) {
     c.lw = __builtin_amdgcn_readlane(words, idx);
     const int off = min(c.lw & kLagMask, SMAX);
-    const float *src = &lds[buf][idx][off + lane];
+    const float *src = &win(buf)[idx][off + lane];
 #pragma unroll
     for (int r = 0; r < R; ++r) c.v[r] = src[64 * r];
     c.w = meta[buf].w[idx][k];
