@@ -303,6 +303,57 @@ def adjoint_timing(log):
     return out
 
 
+def adjoint_brdf_timing(log):
+    """The same for BRDF rows of no diffuse + specular form: the 10x8x3 m
+    shoebox with one directional-scattering BRDF per wall (the same 1,072
+    patches, source and receivers), gradient with respect to the whole
+    BRDF table on the grouped plan (BRDFExchangeAdjoint)."""
+    import torch
+    from sparrowpy_amd import DirectionalRadiosityFast, adjoint, scenes
+    from sparrowpy_amd.coordinates import Coordinates, FrequencyData
+    sc = scenes.directional_shoebox(brdf="directional")
+    r = DirectionalRadiosityFast.from_polygon(sc.walls, sc.patch_size)
+    for idx, br in sc.brdf_groups:
+        r.set_wall_brdf(idx, br, sc.directions, sc.directions)
+    r.set_air_attenuation(FrequencyData(sc.air_attenuation, sc.frequencies))
+    r.bake_geometry()
+    adj = adjoint.BRDFExchangeAdjoint(
+        r, Coordinates(*sc.sources[0]), Coordinates(*sc.receivers.T),
+        sc.speed_of_sound, sc.time_resolution, sc.duration, sc.max_order,
+        checkpoint_every=5)
+    g = torch.Generator(device="cuda").manual_seed(0)
+    t0 = adj.table0
+    target = adj.forward(t0 * (1 + 0.1 * torch.rand(
+        t0.shape, generator=g, device="cuda", dtype=t0.dtype))).to(
+            torch.float64)
+    adj.release()
+
+    def once():
+        table = t0.clone().requires_grad_(True)
+        torch.cuda.synchronize()
+        a = time.perf_counter()
+        H = adjoint.receiver_histograms_brdf(adj, table)
+        torch.cuda.synchronize()
+        b = time.perf_counter()
+        adjoint.log_energy_loss(H, target).backward()
+        torch.cuda.synchronize()
+        c = time.perf_counter()
+        return b - a, c - b, float(table.grad.abs().max())
+
+    once()  # warm-up
+    fwd, bwd, gmax = once()
+    log(f"adjoint (general BRDF rows): forward {fwd:.3f} s, backward "
+        f"{bwd:.3f} s")
+    out = {"workload": f"{sc.name}: P={adj.P}, D={adj.D}, B={adj.B}, "
+                       f"T={adj.T}, order {adj.K}, checkpoints every 5, "
+                       f"{adj.R} receivers, {adj.NK} BRDF rows",
+           "forward_s": fwd, "adjoint_s": bwd, "adjoint_over_forward":
+           bwd / fwd, "max_abs_grad": gmax, "plan": "grouped",
+           "directed_pairs": adj.plan.Vd, "groups": adj.plan.n_groups}
+    del adj, r
+    torch.cuda.empty_cache()
+    return out
+
 def bench_inverse(args, log):
     """Inverse materials on the directional shoebox: target histograms from
     the scene's seeded per-patch absorption and scattering, then Adam steps
@@ -843,6 +894,7 @@ def main(argv=None):
         del state, etc, hist, r
         torch.cuda.empty_cache()
         line["adjoint"] = adjoint_timing(log)
+        line["adjoint_brdf"] = adjoint_brdf_timing(log)
     if not args.no_cpu_baseline:
         usable, logical = _host_cores()
         nth = int(os.environ.get("OMP_NUM_THREADS", usable))

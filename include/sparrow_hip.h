@@ -647,6 +647,56 @@ int spr_ds_injection_grad(int64_t P, int64_t D, int64_t B, int64_t T,
                           const int32_t *spec_dir, const float *mu0,
                           double *gA, double *gC, void *stream);
 
+/* Adjoint of the grouped plan (BRDF rows of any form; rtab[k,b,d] =
+ * table[k,d,b]).  nu[G,B,T] (float32) = the transposed spread of mu =
+ * dL/dE_{k+1}[P,D,B,T]: nu[g,b,t] = sum_d rtab[key_g,b,d] mu[j_g,d,b,t]
+ * (float64 sum), j_g the destination of group g (grp_ptr).  T < 2^29.
+ * Adjoint of the spread half of RadiosityFast.py:1121-1144 (as grouped by
+ * spr_plan_grouped_fill); no counterpart in the reference. */
+int spr_grouped_spread_adjoint(int64_t P, int64_t D, int64_t B, int64_t T,
+                               const int64_t *grp_ptr, const int32_t *grp_key,
+                               const double *rtab, const float *mu, float *nu,
+                               void *stream);
+
+/* Per-group table gradient of one order: gg[g,b,d] += sum_t mu[j_g,d,b,t]
+ * gsum[g,b,t] (gg float64 [G,B,DP], DP = spr_padded_directions(D);
+ * accumulated over orders, each element owned by one lane), with gsum the
+ * forward order's group sums (spr_exchange_gather_grouped of E_k) and mu =
+ * dL/dE_{k+1}.  T < 2^29.
+ * Adjoint support; no counterpart in the reference. */
+int spr_grouped_table_grad(int64_t P, int64_t D, int64_t B, int64_t T,
+                           const int64_t *grp_ptr, const float *mu,
+                           const float *gsum, double *gg, void *stream);
+
+/* Transposed gather: muout[r,b,t] = lam[r,b,t] + sum_{q in T(r)}
+ * tscal[b,q] nu[tgrp_q,b,t+tlag_q] (float64 sum in q order; bins past T
+ * add nothing), r = i*D + d a source row, T(r) = [trow_ptr[r],
+ * trow_ptr[r+1]) its plan entries in entry order with their group tgrp,
+ * delay tlag (no flag) and weights tscal[B,Vd].  muout must not alias lam.
+ * Adjoint of the gather half of RadiosityFast.py:1121-1144; no
+ * counterpart in the reference. */
+int spr_grouped_gather_adjoint(int64_t P, int64_t D, int64_t B, int64_t T,
+                               int64_t Vd, const int64_t *trow_ptr,
+                               const int32_t *tgrp, const int32_t *tlag,
+                               const double *tscal, const float *nu,
+                               const float *lam, float *muout, void *stream);
+
+/* gt[key_g,d,b] += gg[g,b,d] over the G groups (float64 atomics): the
+ * table gradient [NK,D,B] from spr_grouped_table_grad's sums.
+ * Adjoint support; no counterpart in the reference. */
+int spr_grouped_key_grad(int64_t G, int64_t D, int64_t B,
+                         const int32_t *grp_key, const double *gg, double *gt,
+                         void *stream);
+
+/* Injection part of the table gradient [NK,D,B] for E_0dir[p,d,b] =
+ * energy0[p,b] table[k0_p,d,b]: gt[k0_p,d,b] += energy0[p,b]
+ * mu0[p,d,b,n_p], n_p = int(d0[p]/c/dt) (float64 atomics).
+ * Adjoint of RadiosityFast.py:988-1034; no counterpart in the reference. */
+int spr_injection_table_grad(int64_t P, int64_t D, int64_t B, int64_t T,
+                             const double *energy0, const double *d0,
+                             double c, double dt, const int32_t *k0,
+                             const float *mu0, double *gt, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -137,6 +137,16 @@ SYMBOLS = {
                                  _vp]),
     "spr_ds_injection_grad": (_int, [_i64, _i64, _i64, _i64, _vp, _vp, _dbl,
                                      _dbl, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "spr_grouped_spread_adjoint": (_int, [_i64, _i64, _i64, _i64, _vp, _vp,
+                                          _vp, _vp, _vp, _vp]),
+    "spr_grouped_table_grad": (_int, [_i64, _i64, _i64, _i64, _vp, _vp, _vp,
+                                      _vp, _vp]),
+    "spr_grouped_gather_adjoint": (_int, [_i64, _i64, _i64, _i64, _i64, _vp,
+                                          _vp, _vp, _vp, _vp, _vp, _vp,
+                                          _vp]),
+    "spr_grouped_key_grad": (_int, [_i64, _i64, _i64, _vp, _vp, _vp, _vp]),
+    "spr_injection_table_grad": (_int, [_i64, _i64, _i64, _i64, _vp, _vp,
+                                        _dbl, _dbl, _vp, _vp, _vp, _vp]),
     "spr_energy_exchange_workspace": (_i64, [_i64, _i64, _i64, _i64, _i64]),
     "spr_energy_exchange": (_int, [_i64, _i64, _i64, _i64, _vp, _vp, _vp,
                                    _vp, _vp, _dbl, _dbl, _i64, _i64, _vp,

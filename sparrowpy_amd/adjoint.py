@@ -26,11 +26,37 @@ from . import _hip, ops
 from ._hip import call, ptr, stream_ptr
 
 __all__ = ["ExchangeAdjoint", "receiver_histograms", "scattering_tables",
-           "specular_factors", "log_energy_loss"]
+           "specular_factors", "log_energy_loss", "BRDFExchangeAdjoint",
+           "receiver_histograms_brdf", "directional_scattering_tables"]
 
 
 def _i32(n, dev):
     return torch.empty(max(int(n), 1), dtype=torch.int32, device=dev)
+
+
+def _source_and_receivers(adj, r, source, receivers):
+    """Set on ``adj`` (with P, D, B set) what both adjoints share: the
+    source position, its energy per patch before the BRDF (``energy0``,
+    ``d0``; reference ``RadiosityFast.py:436-522``), the geometry, the BRDF
+    row maps, and the receivers' collection terms."""
+    b, g = r._bake, r._geometry()
+    pos = np.asarray(source.cartesian if hasattr(source, "cartesian")
+                     else source, dtype=float).reshape(3)
+    adj.source = ops.as_dev(pos, torch.float64)
+    vis = ops.check_point2patch_visibility(
+        pos, g["centers"], g["walls_normal"], g["walls_points"])[0]
+    e0, _, d0 = ops.source_energy(
+        pos, g["centers"], g["points"], vis, b["air"], adj.B,
+        g["wall_ids"], b["sources"], b["brdf"], b["brdf_index"], adj.D)
+    adj.energy0, adj.d0 = e0, d0
+    adj._geom = g
+    adj.brdf_index = ops.as_dev(b["brdf_index"], torch.int64)
+    adj.in_dirs = ops.as_dev(b["sources"], torch.float64)
+    pts, _, adj.rfactor, adj.ridx = r._receiver_terms(receivers)
+    adj.rpts = ops.as_dev(pts, torch.float64)
+    adj.air = ops.as_dev(b["air"] if b["air"] is not None
+                         else np.zeros(adj.B), torch.float64)
+    adj.R = adj.rpts.shape[0]
 
 
 class ExchangeAdjoint:
@@ -71,29 +97,11 @@ class ExchangeAdjoint:
                                              self.NK)
         self.A0, self.C0, self.spec_dir = tables
         self._transpose()
-        # injection before the BRDF, and the source-row map
-        pos = np.asarray(source.cartesian if hasattr(source, "cartesian")
-                         else source, dtype=float).reshape(3)
-        self.source = ops.as_dev(pos, torch.float64)
-        vis = ops.check_point2patch_visibility(
-            pos, g["centers"], g["walls_normal"], g["walls_points"])[0]
-        e0, _, d0 = ops.source_energy(
-            pos, g["centers"], g["points"], vis, b["air"], self.B,
-            g["wall_ids"], b["sources"], b["brdf"], b["brdf_index"], self.D)
-        self.energy0, self.d0 = e0, d0
+        _source_and_receivers(self, r, source, receivers)
         self.k0 = _i32(self.P, dev)
         self.e0dir = torch.empty((self.P, self.D, self.B), dtype=torch.float64,
                                  device=dev)
         self.S = adj["S"]
-        self._geom = g
-        self.brdf_index = ops.as_dev(b["brdf_index"], torch.int64)
-        self.in_dirs = ops.as_dev(b["sources"], torch.float64)
-        # receivers
-        pts, _, self.rfactor, self.ridx = r._receiver_terms(receivers)
-        self.rpts = ops.as_dev(pts, torch.float64)
-        self.air = ops.as_dev(b["air"] if b["air"] is not None
-                              else np.zeros(self.B), torch.float64)
-        self.R = self.rpts.shape[0]
 
     # -- plan views ----------------------------------------------------------
     def _transpose(self):
@@ -265,6 +273,268 @@ def receiver_histograms(adj, A, C):
     """Receiver histograms [R, B, T] (float64 tensor) as a differentiable
     function of the tables ``A``, ``C`` ([NK, B] float64 device tensors)."""
     return _Histograms.apply(A, C, adj)
+
+
+class BRDFExchangeAdjoint:
+    """Forward and adjoint of ``BRDF table -> receiver histograms`` for BRDF
+    rows of any form (``create_from_directional_scattering`` among them,
+    reference ``brdf.py:133-224``), on the grouped plan.
+
+    The table is ``table[k, d, b]`` with row ``k = brdf * S + s`` as
+    ``set_wall_brdf`` stores it (``brdf.freq * pi``; :attr:`table0` is the
+    radiosity object's own); one order is ``gsum = gather(E_k)`` per group
+    of sources sharing a BRDF row, ``E_{k+1}[j, d] = sum_g table[key_g, d]
+    gsum[g]``, and injection puts ``energy0[p, b] table[k0_p, :, b]`` at
+    each patch's delay bin (``RadiosityFast.py:988-1145``).  The backward
+    runs the transposed orders from the last (``spr_grouped_spread_adjoint``,
+    ``spr_grouped_gather_adjoint``), the forward orders recomputed from
+    checkpoints kept every ``checkpoint_every`` orders, and sums the table
+    gradient per group and order (``spr_grouped_table_grad``), then per
+    BRDF row.  Arguments as for :class:`ExchangeAdjoint`.
+    """
+
+    def __init__(self, radiosity, source, receivers, speed_of_sound,
+                 time_resolution, duration, max_order, checkpoint_every=5):
+        r = radiosity
+        if r._bake is None:
+            raise RuntimeError("bake_geometry() must run first")
+        b, g = r._bake, r._geometry()
+        if b["brdf"] is None:
+            raise ValueError("the adjoint needs BRDF tables")
+        self.c, self.dt = float(speed_of_sound), float(time_resolution)
+        self.T = int(duration / time_resolution)
+        self.K = int(max_order)
+        self.every = max(1, int(checkpoint_every))
+        self.last = None   # TableRecord of the last forward()
+        self.plan = ops.GroupedPlan.from_geometry(
+            b["vis"], g["centers"], g["areas"], b["ff"], b["p2o"], b["air"],
+            g["wall_ids"], b["sources"], b["brdf"], b["brdf_index"], b["D"],
+            b["n_bins"], self.c, self.dt)
+        p = self.plan
+        if r.TILE_ROWS:
+            # the class's destination tiles: the same group sums, faster
+            p.tiles = ops.tile_plan(p, r.TILE_ROWS, r.TILE_MIN_MERGE,
+                                    r._tile_order())
+            if p.tiles is not None:
+                p.tiles.R = r.TILE_R
+        self.P, self.D, self.B = p.P, p.D, p.B
+        self.dev = dev = g["centers"].device
+        brdf = ops.as_dev(b["brdf"], torch.float64)
+        self.S = int(brdf.shape[1])
+        self.NK = int(brdf.shape[0]) * self.S
+        self.table0 = brdf.reshape(self.NK, self.D, self.B).clone()
+        self._transpose()
+        _source_and_receivers(self, r, source, receivers)
+        # each patch's BRDF row for the direction of the source
+        # (spr_ds_injection with zero tables fills k0 only)
+        self.k0 = _i32(self.P, dev)
+        zero = torch.zeros((self.NK, self.B), dtype=torch.float64, device=dev)
+        spec = torch.zeros(self.NK, dtype=torch.int32, device=dev)
+        scratch = torch.empty((self.P, self.D, self.B), dtype=torch.float64,
+                              device=dev)
+        call("spr_ds_injection", self.P, self.D, self.B, ptr(self.source),
+             ptr(g["centers"]), ptr(g["wall_ids"]), self.S,
+             ptr(self.brdf_index), ptr(self.in_dirs), ptr(self.energy0),
+             ptr(zero), ptr(zero), ptr(spec), ptr(self.k0), ptr(scratch),
+             stream_ptr())
+        self._k0 = self.k0.to(torch.int64)
+
+    def _transpose(self):
+        """Entries by source row r = i * D + d, in entry order: trow_ptr
+        [P*D+1], each entry's group, delay (flag cleared) and weights."""
+        p, dev = self.plan, self.dev
+        Vd, G, PD = p.Vd, p.n_groups, self.P * self.D
+        rowid = p.rowid.to(torch.int64)
+        order = torch.sort(rowid, stable=True).indices
+        self.trow_ptr = torch.zeros(PD + 1, dtype=torch.int64, device=dev)
+        if Vd:
+            self.trow_ptr[1:] = torch.cumsum(
+                torch.bincount(rowid, minlength=PD), 0)
+        gid = torch.repeat_interleave(
+            torch.arange(G, dtype=torch.int64, device=dev),
+            p.grp_start[1:G + 1] - p.grp_start[:G])
+        self.tgrp = _i32(Vd, dev)
+        self.tlag = _i32(Vd, dev)
+        self.tscal = torch.empty((self.B, max(Vd, 1)), dtype=torch.float64,
+                                 device=dev)
+        if Vd:
+            self.tgrp[:Vd] = gid[order].to(torch.int32)
+            self.tlag[:Vd] = p.lag[order] & 0x3fffffff
+            self.tscal[:, :Vd] = p.scal[:, order]
+
+    def _set_table(self, table):
+        t = ops.as_dev(table, torch.float64)
+        if tuple(t.shape) != (self.NK, self.D, self.B):
+            raise ValueError(f"table must be ({self.NK}, {self.D}, {self.B})")
+        self.plan.rtab[:, :, :self.D] = t.permute(0, 2, 1)
+        self.e0dir = (self.energy0[:, None, :] * t[self._k0]).contiguous()
+
+    def _gather(self, state, gsum):
+        ops.gather_groups(self.plan, (self.P, self.D, self.B, self.T), state,
+                          gsum)
+
+    def _order(self, state_prev, out, gsum):
+        """One order without the running total: out = A state_prev."""
+        p = self.plan
+        self._gather(state_prev, gsum)
+        call("spr_exchange_spread_grouped", self.P, self.D, self.B, self.T,
+             ptr(p.grp_ptr), ptr(p.grp_key), ptr(p.rtab), ptr(gsum),
+             ptr(out), None, None, stream_ptr())
+
+    def forward(self, table):
+        """Receiver histograms (float32 [R, B, T]) for ``table``; the call's
+        checkpoints and table are kept as ``self.last`` (see
+        :meth:`ExchangeAdjoint.forward`)."""
+        table = ops.as_dev(table, torch.float64).detach().clone()
+        self._set_table(table)
+        st = ops.ExchangeState(self.P, self.D, self.B, self.T, self.dev)
+        st.init(self.e0dir, self.d0, self.c, self.dt)
+        checkpoints = {0: st.prev.clone()}
+        for k in range(self.K):
+            st.step(self.plan)
+            if (k + 1) % self.every == 0 and k + 1 < self.K:
+                checkpoints[k + 1] = st.prev.clone()
+        st.check()
+        self.last = TableRecord(table, checkpoints)
+        total = st.finalize(out=st.hi)
+        H = ops.collect(total, self.rpts, self._geom["centers"], self.ridx,
+                        self.rfactor, self.air, self.c, self.dt)
+        del st, total
+        return H
+
+    def backward(self, grad_hist, record=None):
+        """dL/dtable (float64 [NK, D, B]) for dL/dH = ``grad_hist`` ([R, B,
+        T]) at the table of ``record`` (default: the last :meth:`forward`).
+        """
+        record = self.last if record is None else record
+        if record is None or not record.checkpoints:
+            raise RuntimeError("BRDFExchangeAdjoint.backward: no forward "
+                               "record (forward() was not called, or its "
+                               "record was released)")
+        self._set_table(record.table)
+        checkpoints = record.checkpoints
+        starts = sorted(checkpoints)
+        if starts != list(range(0, max(self.K, 1), self.every)):
+            raise RuntimeError("BRDFExchangeAdjoint.backward: the record's "
+                               "checkpoints do not cover the orders")
+        P, D, B, T = self.P, self.D, self.B, self.T
+        p, dev = self.plan, self.dev
+        G = p.n_groups
+        gH = ops.as_dev(grad_hist, torch.float32).contiguous()
+        shape = (P, D, B, T)
+        lam = torch.empty(shape, dtype=torch.float32, device=dev)
+        call("spr_collect_adjoint", self.R, P, D, B, T, ptr(gH),
+             ptr(self.rpts), ptr(self._geom["centers"]), ptr(self.ridx),
+             ptr(self.rfactor), ptr(self.air), self.c, self.dt, ptr(lam),
+             stream_ptr())
+        DP = p.rtab.shape[2]
+        gg = torch.zeros((max(G, 1), B, DP), dtype=torch.float64, device=dev)
+        gsum = torch.empty((max(G, 1), B, T), dtype=torch.float32, device=dev)
+        nu = torch.empty_like(gsum)
+        mu = lam.clone()                     # mu_K
+        mu_out = torch.empty_like(lam)
+        for s in reversed(starts):
+            end = min(s + self.every, self.K)
+            states = [checkpoints[s]]        # E_s .. E_{end-1}
+            # the group sums of orders s .. end-2 come out of recomputing
+            # the segment; kept (when there is room) instead of gathered
+            # again for the backward orders
+            keep = _fits(gsum, end - s - 1)
+            sums = {}
+            for k in range(s, end - 1):
+                nxt = torch.empty_like(lam)
+                gk = torch.empty_like(gsum) if keep else gsum
+                self._order(states[-1], nxt, gk)
+                if keep:
+                    sums[k] = gk
+                states.append(nxt)
+            for k in range(end - 1, s - 1, -1):
+                # mu = dL/dE_{k+1}; the group sums of order k -> k+1
+                gk = sums.pop(k, None)
+                if gk is None:
+                    gk = gsum
+                    self._gather(states[k - s], gk)
+                call("spr_grouped_table_grad", P, D, B, T, ptr(p.grp_ptr),
+                     ptr(mu), ptr(gk), ptr(gg), stream_ptr())
+                del gk
+                call("spr_grouped_spread_adjoint", P, D, B, T,
+                     ptr(p.grp_ptr), ptr(p.grp_key), ptr(p.rtab), ptr(mu),
+                     ptr(nu), stream_ptr())
+                call("spr_grouped_gather_adjoint", P, D, B, T, p.Vd,
+                     ptr(self.trow_ptr), ptr(self.tgrp), ptr(self.tlag),
+                     ptr(self.tscal), ptr(nu), ptr(lam), ptr(mu_out),
+                     stream_ptr())
+                mu, mu_out = mu_out, mu
+            del states
+        gt = torch.zeros((self.NK, D, B), dtype=torch.float64, device=dev)
+        call("spr_grouped_key_grad", G, D, B, ptr(p.grp_key), ptr(gg),
+             ptr(gt), stream_ptr())
+        call("spr_injection_table_grad", P, D, B, T, ptr(self.energy0),
+             ptr(self.d0), self.c, self.dt, ptr(self.k0), ptr(mu), ptr(gt),
+             stream_ptr())
+        return gt
+
+    def release(self):
+        """Drop the record of the last forward."""
+        self.last = None
+
+
+def _fits(like, n):
+    """Whether ``n`` more tensors like ``like`` fit in half of the device
+    memory still available (free plus the allocator's unused cache)."""
+    if n <= 0:
+        return False
+    free, _ = torch.cuda.mem_get_info(like.device)
+    idle = torch.cuda.memory_reserved(like.device) - \
+        torch.cuda.memory_allocated(like.device)
+    return n * like.numel() * like.element_size() < 0.5 * (free + idle)
+
+
+class TableRecord:
+    """What the adjoint of one BRDFExchangeAdjoint forward needs: its table
+    and the order histograms kept every ``checkpoint_every`` orders."""
+
+    def __init__(self, table, checkpoints):
+        self.table, self.checkpoints = table, checkpoints
+
+
+class _TableHistograms(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, table, adj):
+        ctx.adj = adj
+        H = adj.forward(table.detach()).to(torch.float64)
+        ctx.record, adj.last = adj.last, None
+        return H
+
+    @staticmethod
+    def backward(ctx, grad):
+        return ctx.adj.backward(grad, ctx.record), None
+
+
+def receiver_histograms_brdf(adj, table):
+    """Receiver histograms [R, B, T] (float64 tensor) as a differentiable
+    function of the BRDF table ``table`` ([NK, D, B] float64 device tensor,
+    rows as :attr:`BRDFExchangeAdjoint.table0`)."""
+    return _TableHistograms.apply(table, adj)
+
+
+def directional_scattering_tables(alpha, directional_scattering,
+                                  receiver_directions):
+    """BRDF tables of ``create_from_directional_scattering`` (reference
+    ``brdf.py:133-224``) as the radiosity class stores them (times pi):
+    ``table[n*S + s, d, b] = pi (1 - alpha[n, b]) sd[n, s, d, b] /
+    (w_d cos(theta_d))``, with the outgoing weights normalised to 2 pi.
+    Differentiable in ``alpha`` ([N, B]) and ``directional_scattering``
+    ([N, S, D, B]); returns [N*S, D, B]."""
+    from .brdf import _colatitude
+    cart = np.asarray(receiver_directions.cartesian, float)
+    w = np.array(receiver_directions.weights, dtype=float)
+    w = w * 2 * np.pi / w.sum()
+    scale = np.pi / (w * np.cos(_colatitude(cart)))         # [D]
+    sd = directional_scattering
+    f = torch.as_tensor(scale, dtype=sd.dtype, device=sd.device)
+    t = (1.0 - alpha)[:, None, None, :] * sd * f[None, None, :, None]
+    return t.reshape(-1, t.shape[2], t.shape[3])
 
 
 def specular_factors(directions):

@@ -277,10 +277,221 @@ __global__ void injection_grad_kernel(
   if (m < D) unsafeAtomicAdd(gC + k * B + b, e * sm);
 }
 
+// ---------------------------------------------------------------------
+// Adjoint of the grouped plan (any BRDF rows)
+// ---------------------------------------------------------------------
+// One forward order on the grouped plan (exchange.hip, RadiosityFast.py:
+// 1114-1143 grouped by BRDF row):
+//   gsum[g,b,t] = sum_{e in g} scal[b,e] E_k[row_e, b, t - n_e],
+//   E_{k+1}[j,d,b,t] = sum_{g of j} rtab[key_g, b, d] gsum[g,b,t],
+// with rtab[k,b,d] = table[k,d,b] (BRDF row k = brdf * S + s).  For
+// mu = dL/dE_{k+1}:
+//   nu[g,b,t] = sum_d rtab[key_g,b,d] mu[j_g,d,b,t]              (spread^T)
+//   dL/dE_k[r,b,t] += sum_{e: row_e = r} scal[b,e] nu[g_e,b,t+n_e] (gather^T)
+//   dL/dtable[key_g,d,b] += sum_t mu[j_g,d,b,t] gsum[g,b,t].
+
+// nu for every group of destination j: a workgroup per (j, band, 256
+// bins), one bin per lane; the lane keeps mu[j,:,b,t] as DPT doubles
+// (converted once), and the workgroup stages its groups' BRDF rows in LDS,
+// 16 at a time, read by broadcast.
+template <int DPT>
+__global__ __launch_bounds__(256) void grouped_spread_adjoint_kernel(
+    int D, int B, int T, int DP, int n_tblk,
+    const int64_t *__restrict__ grp_ptr, const int32_t *__restrict__ grp_key,
+    const double *__restrict__ rtab, const float *__restrict__ mu,
+    float *__restrict__ nu) {
+  constexpr int NG = 16;
+  __shared__ double rts[NG][DPT];
+  const int64_t blk = blockIdx.x;
+  const int tb = (int)(blk % n_tblk);
+  const int b = (int)((blk / n_tblk) % B);
+  const int64_t j = blk / ((int64_t)n_tblk * B);
+  const int64_t g0 = grp_ptr[j], g1 = grp_ptr[j + 1];
+  if (g0 == g1) return;  // uniform
+  const int t = tb * 256 + (int)threadIdx.x;
+  const bool in = t < T;
+  const int64_t dstride = (int64_t)B * T;
+  const float *m = mu + (j * D * B + b) * (int64_t)T + (in ? t : 0);
+  double v[DPT];
+#pragma unroll
+  for (int d = 0; d < DPT; ++d)
+    v[d] = (d < D && in) ? (double)m[d * dstride] : 0.0;
+  for (int64_t gc = g0; gc < g1; gc += NG) {
+    const int ng = (int)min((int64_t)NG, g1 - gc);
+    for (int idx = (int)threadIdx.x; idx < NG * DPT; idx += 256) {
+      const int gi = idx / DPT, d = idx % DPT;
+      rts[gi][d] = (gi < ng && d < DP)
+                       ? rtab[((int64_t)grp_key[gc + gi] * B + b) * DP + d]
+                       : 0.0;
+    }
+    __syncthreads();
+    for (int gi = 0; gi < ng; ++gi) {
+      double s = 0.0;
+#pragma unroll
+      for (int d = 0; d < DPT; ++d) s = fma(rts[gi][d], v[d], s);
+      if (in) nu[((gc + gi) * B + b) * (int64_t)T + t] = (float)s;
+    }
+    __syncthreads();
+  }
+}
+
+// gg[g,b,d] += sum_t mu[j_g,d,b,t] gsum[g,b,t] for the groups g of
+// destination j: per (j, band) the product of the group sums [G_j x T]
+// with mu[j,:,b,:]^T [T x D], in tiles of 64 groups x 64 directions, the
+// bins in chunks of 32 staged in LDS (float32: staged as float64 the
+// launch took 16.2 ms against 15.2 on the directional shoebox).  Each lane
+// sums 4 x 4 (group, direction) pairs in float64 and owns them: no
+// atomics, and the result does not depend on the launch's timing.
+__global__ __launch_bounds__(256) void grouped_table_grad_kernel(
+    int D, int B, int T, int DP, const int64_t *__restrict__ grp_ptr,
+    const float *__restrict__ mu, const float *__restrict__ gsum,
+    double *__restrict__ gg) {
+  constexpr int TC = 32;
+  __shared__ float sg[64][TC + 1];
+  __shared__ float sm[64][TC + 1];
+  const int b = (int)(blockIdx.x % B);
+  const int64_t j = blockIdx.x / B;
+  const int64_t g0 = grp_ptr[j], g1 = grp_ptr[j + 1];
+  const int tid = (int)threadIdx.x;
+  const int tg = tid >> 4, td = tid & 15;
+  const int64_t dstride = (int64_t)B * T;
+  const float *mrow = mu + (j * D * B + b) * (int64_t)T;  // (j, 0, b, 0)
+  for (int64_t gt = g0; gt < g1; gt += 64) {
+    double acc[4][4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) acc[q][r] = 0.0;
+    for (int t0 = 0; t0 < T; t0 += TC) {
+#pragma unroll
+      for (int s = 0; s < 64 * TC / 256; ++s) {
+        const int idx = tid + 256 * s;
+        const int row = idx / TC, col = idx % TC;
+        const int t = t0 + col;
+        const int64_t g = gt + row;
+        sg[row][col] =
+            (g < g1 && t < T) ? gsum[(g * B + b) * (int64_t)T + t] : 0.f;
+        sm[row][col] = (row < D && t < T) ? mrow[row * dstride + t] : 0.f;
+      }
+      __syncthreads();
+#pragma unroll 4
+      for (int c = 0; c < TC; ++c) {
+        double a[4], m[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) a[q] = (double)sg[tg + 16 * q][c];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) m[r] = (double)sm[td + 16 * r][c];
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) acc[q][r] = fma(a[q], m[r], acc[q][r]);
+      }
+      __syncthreads();
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int64_t g = gt + tg + 16 * q;
+        const int d = td + 16 * r;
+        if (g < g1 && d < D) gg[(g * B + b) * DP + d] += acc[q][r];
+      }
+  }
+}
+
+// muout[r,b,t] = lam[r,b,t] + sum_{q in T(r)} tscal[b,q] nu[tgrp_q,b,t+n_q]
+// (float64 sum in transposed-entry order; a bin t + n beyond the histogram
+// reads 0 through the row's buffer descriptor and adds nothing).  A
+// workgroup per (source row r, band, 1,024 bins), 4 bins per lane, rows
+// taken direction-major (r = i*D + d for d slow, i fast: neighbouring
+// patches sending in one direction feed the same groups, whose nu rows
+// they then share through L2); the entries are taken C at a time, every
+// load of the chunk issued before any is summed.
+__global__ __launch_bounds__(256) void grouped_gather_adjoint_kernel(
+    int64_t P, int D, int B, int T, int n_tblk, int64_t Vd,
+    const int64_t *__restrict__ trow_ptr,
+    const int32_t *__restrict__ tgrp, const int32_t *__restrict__ tlag,
+    const double *__restrict__ tscal, const float *__restrict__ nu,
+    const float *__restrict__ lam, float *__restrict__ muout) {
+  constexpr int R = 4, C = 4;
+  const int64_t blk = blockIdx.x;
+  const int tb = (int)(blk % n_tblk);
+  const int b = (int)((blk / n_tblk) % B);
+  const int64_t rd = blk / ((int64_t)n_tblk * B);  // d * P + i
+  const int64_t r = (rd % P) * D + rd / P;
+  const int64_t q0 = trow_ptr[r], q1 = trow_ptr[r + 1];
+  const int64_t base = (r * B + b) * (int64_t)T;
+  const uint32_t row_bytes = (uint32_t)T * 4u;
+  int t[R];
+  double s[R];
+#pragma unroll
+  for (int i = 0; i < R; ++i) {
+    t[i] = tb * 256 * R + (int)threadIdx.x + 256 * i;
+    s[i] = t[i] < T ? (double)lam[base + t[i]] : 0.0;
+  }
+  for (int64_t q = q0; q < q1; q += C) {
+    float v[C][R];
+    double w[C];
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+      const bool ok = q + c < q1;  // uniform
+      const int64_t qq = ok ? q + c : q0;
+      const int n = tlag[qq];
+      w[c] = ok ? tscal[b * Vd + qq] : 0.0;
+      const __amdgpu_buffer_rsrc_t rs = make_rsrc(
+          nu + ((int64_t)tgrp[qq] * B + b) * (int64_t)T, ok ? row_bytes : 0u);
+#pragma unroll
+      for (int i = 0; i < R; ++i) v[c][i] = buffer_load(rs, bin_offset(t[i] + n));
+    }
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+      if (q + c >= q1) break;  // uniform
+#pragma unroll
+      for (int i = 0; i < R; ++i) s[i] = fma(w[c], (double)v[c][i], s[i]);
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < R; ++i)
+    if (t[i] < T) muout[base + t[i]] = (float)s[i];
+}
+
+// gt[key_g,d,b] += gg[g,b,d] (float64 atomics: the groups of one BRDF row
+// add in no fixed order).
+__global__ void grouped_key_grad_kernel(int64_t G, int D, int B, int DP,
+                                        const int32_t *__restrict__ grp_key,
+                                        const double *__restrict__ gg,
+                                        double *__restrict__ gt) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= G * B * D) return;
+  const int d = (int)(q % D);
+  const int b = (int)((q / D) % B);
+  const int64_t g = q / ((int64_t)D * B);
+  unsafeAtomicAdd(gt + ((int64_t)grp_key[g] * D + d) * B + b,
+                  gg[(g * B + b) * DP + d]);
+}
+
+// Injection part: E_0dir[p,d,b] = energy0[p,b] table[k0_p,d,b], placed at
+// bin n_p, so gt[k0_p,d,b] += energy0[p,b] mu0[p,d,b,n_p].
+__global__ void injection_table_grad_kernel(
+    int64_t P, int64_t D, int64_t B, int64_t T,
+    const double *__restrict__ energy0, const double *__restrict__ d0,
+    double c, double dt, const int32_t *__restrict__ k0,
+    const float *__restrict__ mu0, double *__restrict__ gt) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= P * D * B) return;
+  const int64_t b = q % B, d = (q / B) % D, p = q / (D * B);
+  const int64_t n = delay_floor(d0[p], c, dt);
+  if (n < 0 || n >= T) return;
+  unsafeAtomicAdd(gt + ((int64_t)k0[p] * D + d) * B + b,
+                  energy0[p * B + b] * (double)mu0[q * T + n]);
+}
+
 }  // namespace
 }  // namespace spr
 
 using namespace spr;
+
+extern "C" int64_t spr_padded_directions(int64_t D);  // exchange.hip
 
 extern "C" int spr_ds_weights(int64_t B, int64_t Vd, const int32_t *krow,
                               const double *wgt, const double *a,
@@ -413,6 +624,114 @@ extern "C" int spr_ds_injection_grad(int64_t P, int64_t D, int64_t B,
                      dim3((unsigned)ceil_div(P * B, 256)), dim3(256), 0,
                      as_stream(stream), P, D, B, T, energy0, d0, c, dt, k0,
                      spec_dir, mu0, gA, gC);
+  SPR_LAUNCH_CHECK();
+  return SPR_OK;
+}
+
+extern "C" int spr_grouped_spread_adjoint(int64_t P, int64_t D, int64_t B,
+                                          int64_t T, const int64_t *grp_ptr,
+                                          const int32_t *grp_key,
+                                          const double *rtab, const float *mu,
+                                          float *nu, void *stream) {
+  const int64_t DP = spr_padded_directions(D);
+  SPR_REQUIRE(P > 0 && DP > 0 && B > 0 && T > 0 && T < ((int64_t)1 << 29),
+              "spr_grouped_spread_adjoint: bad sizes");
+  SPR_REQUIRE(grp_ptr && grp_key && rtab && mu && nu,
+              "spr_grouped_spread_adjoint: null pointer");
+  const int64_t nt = ceil_div(T, (int64_t)256);
+  const int64_t nb = P * B * nt;
+  SPR_REQUIRE(nb < (int64_t)0x7fffffff,
+              "spr_grouped_spread_adjoint: grid too large");
+  hipStream_t s = as_stream(stream);
+  if (DP <= 8)
+    hipLaunchKernelGGL((grouped_spread_adjoint_kernel<8>), dim3((unsigned)nb),
+                       dim3(256), 0, s, (int)D, (int)B, (int)T, (int)DP,
+                       (int)nt, grp_ptr, grp_key, rtab, mu, nu);
+  else if (DP <= 16)
+    hipLaunchKernelGGL((grouped_spread_adjoint_kernel<16>), dim3((unsigned)nb),
+                       dim3(256), 0, s, (int)D, (int)B, (int)T, (int)DP,
+                       (int)nt, grp_ptr, grp_key, rtab, mu, nu);
+  else if (DP <= 32)
+    hipLaunchKernelGGL((grouped_spread_adjoint_kernel<32>), dim3((unsigned)nb),
+                       dim3(256), 0, s, (int)D, (int)B, (int)T, (int)DP,
+                       (int)nt, grp_ptr, grp_key, rtab, mu, nu);
+  else
+    hipLaunchKernelGGL((grouped_spread_adjoint_kernel<64>), dim3((unsigned)nb),
+                       dim3(256), 0, s, (int)D, (int)B, (int)T, (int)DP,
+                       (int)nt, grp_ptr, grp_key, rtab, mu, nu);
+  SPR_LAUNCH_CHECK();
+  return SPR_OK;
+}
+
+extern "C" int spr_grouped_table_grad(int64_t P, int64_t D, int64_t B,
+                                      int64_t T, const int64_t *grp_ptr,
+                                      const float *mu, const float *gsum,
+                                      double *gg, void *stream) {
+  const int64_t DP = spr_padded_directions(D);
+  SPR_REQUIRE(P > 0 && DP > 0 && B > 0 && T > 0 && T < ((int64_t)1 << 29),
+              "spr_grouped_table_grad: bad sizes");
+  SPR_REQUIRE(grp_ptr && mu && gsum && gg,
+              "spr_grouped_table_grad: null pointer");
+  SPR_REQUIRE(P * B < (int64_t)0x7fffffff,
+              "spr_grouped_table_grad: grid too large");
+  hipLaunchKernelGGL(grouped_table_grad_kernel, dim3((unsigned)(P * B)),
+                     dim3(256), 0, as_stream(stream), (int)D, (int)B, (int)T,
+                     (int)DP, grp_ptr, mu, gsum, gg);
+  SPR_LAUNCH_CHECK();
+  return SPR_OK;
+}
+
+extern "C" int spr_grouped_gather_adjoint(
+    int64_t P, int64_t D, int64_t B, int64_t T, int64_t Vd,
+    const int64_t *trow_ptr, const int32_t *tgrp, const int32_t *tlag,
+    const double *tscal, const float *nu, const float *lam, float *muout,
+    void *stream) {
+  SPR_REQUIRE(P > 0 && D > 0 && B > 0 && T > 0 && Vd >= 0 &&
+                  T < ((int64_t)1 << 29),
+              "spr_grouped_gather_adjoint: bad sizes");
+  SPR_REQUIRE(trow_ptr && tgrp && tlag && tscal && nu && lam && muout,
+              "spr_grouped_gather_adjoint: null pointer");
+  SPR_REQUIRE(muout != lam, "spr_grouped_gather_adjoint: muout aliases lam");
+  const int64_t nt = ceil_div(T, (int64_t)1024);
+  const int64_t nb = P * D * B * nt;
+  SPR_REQUIRE(nb < (int64_t)0x7fffffff,
+              "spr_grouped_gather_adjoint: grid too large");
+  hipLaunchKernelGGL(grouped_gather_adjoint_kernel, dim3((unsigned)nb),
+                     dim3(256), 0, as_stream(stream), P, (int)D, (int)B,
+                     (int)T, (int)nt, Vd, trow_ptr, tgrp, tlag, tscal, nu,
+                     lam, muout);
+  SPR_LAUNCH_CHECK();
+  return SPR_OK;
+}
+
+extern "C" int spr_grouped_key_grad(int64_t G, int64_t D, int64_t B,
+                                    const int32_t *grp_key, const double *gg,
+                                    double *gt, void *stream) {
+  const int64_t DP = spr_padded_directions(D);
+  SPR_REQUIRE(G >= 0 && DP > 0 && B > 0, "spr_grouped_key_grad: bad sizes");
+  SPR_REQUIRE(grp_key && gg && gt, "spr_grouped_key_grad: null pointer");
+  if (G == 0) return SPR_OK;
+  hipLaunchKernelGGL(grouped_key_grad_kernel,
+                     dim3((unsigned)ceil_div(G * B * D, 256)), dim3(256), 0,
+                     as_stream(stream), G, (int)D, (int)B, (int)DP, grp_key,
+                     gg, gt);
+  SPR_LAUNCH_CHECK();
+  return SPR_OK;
+}
+
+extern "C" int spr_injection_table_grad(int64_t P, int64_t D, int64_t B,
+                                        int64_t T, const double *energy0,
+                                        const double *d0, double c, double dt,
+                                        const int32_t *k0, const float *mu0,
+                                        double *gt, void *stream) {
+  SPR_REQUIRE(P > 0 && D > 0 && B > 0 && T > 0 && c > 0 && dt > 0,
+              "spr_injection_table_grad: bad sizes");
+  SPR_REQUIRE(energy0 && d0 && k0 && mu0 && gt,
+              "spr_injection_table_grad: null pointer");
+  hipLaunchKernelGGL(injection_table_grad_kernel,
+                     dim3((unsigned)ceil_div(P * D * B, 256)), dim3(256), 0,
+                     as_stream(stream), P, D, B, T, energy0, d0, c, dt, k0,
+                     mu0, gt);
   SPR_LAUNCH_CHECK();
   return SPR_OK;
 }

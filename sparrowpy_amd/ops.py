@@ -773,6 +773,41 @@ class StatePool:
         return self.bufs[k][:int(n)]
 
 
+def gather_groups(plan, shape, prev, gsum):
+    """Pass 1 of an order on a GroupedPlan: the per-group sums of ``prev``
+    ([P, D, B, T] float32) into ``gsum`` ([G, B, T] float32), by the plan's
+    destination tiles when it has them (the same sums, bit for bit)."""
+    P, D, B, T = shape
+    if plan.diffuse_specular and plan.tiles is not None:
+        tp = plan.tiles
+        call("spr_exchange_gather_tiled_ds", tp.P, tp.JT, B, T, tp.M,
+             ptr(tp.tptr), ptr(tp.gstart), ptr(tp.gdiff), ptr(tp.mspan),
+             ptr(tp.mlag), ptr(tp.mw), ptr(tp.mdw), ptr(prev), ptr(gsum),
+             stream_ptr())
+        return
+    if plan.diffuse_specular:
+        call("spr_exchange_gather_ds", P, B, T, plan.Vd,
+             ptr(plan.row_ptr), ptr(plan.grp_ptr), ptr(plan.rowid),
+             ptr(plan.lag), ptr(plan.scal), ptr(plan.dscal), ptr(prev),
+             ptr(gsum), stream_ptr())
+        return
+    if plan.tiles is not None:
+        tp = plan.tiles
+        if tp.JT == 16:
+            call("spr_exchange_gather_tiled_shape", tp.P, tp.JT, tp.R,
+                 B, T, tp.M, ptr(tp.tptr), ptr(tp.gstart),
+                 ptr(tp.mspan), ptr(tp.mlag), ptr(tp.mw), ptr(prev),
+                 ptr(gsum), stream_ptr())
+            return
+        call("spr_exchange_gather_tiled", tp.P, tp.JT, B, T, tp.M,
+             ptr(tp.tptr), ptr(tp.gstart), ptr(tp.mrow),
+             ptr(tp.mlag), ptr(tp.mw), ptr(prev), ptr(gsum), stream_ptr())
+        return
+    call("spr_exchange_gather_grouped", P, B, T, plan.Vd,
+         ptr(plan.row_ptr), ptr(plan.grp_ptr), ptr(plan.rowid),
+         ptr(plan.lag), ptr(plan.scal), ptr(prev), ptr(gsum), stream_ptr())
+
+
 class ExchangeState:
     """Order histograms and the compensated running total of one source.
 
@@ -842,37 +877,7 @@ class ExchangeState:
 
     def gather(self, plan):
         """Grouped plan, pass 1 of an order (per-group sums)."""
-        P, D, B, T = self.shape
-        if plan.diffuse_specular and plan.tiles is not None:
-            tp = plan.tiles
-            call("spr_exchange_gather_tiled_ds", tp.P, tp.JT, B, T, tp.M,
-                 ptr(tp.tptr), ptr(tp.gstart), ptr(tp.gdiff), ptr(tp.mspan),
-                 ptr(tp.mlag), ptr(tp.mw), ptr(tp.mdw), ptr(self.prev),
-                 ptr(self._gsum(plan)), stream_ptr())
-            return
-        if plan.diffuse_specular:
-            call("spr_exchange_gather_ds", P, B, T, plan.Vd,
-                 ptr(plan.row_ptr), ptr(plan.grp_ptr), ptr(plan.rowid),
-                 ptr(plan.lag), ptr(plan.scal), ptr(plan.dscal),
-                 ptr(self.prev), ptr(self._gsum(plan)), stream_ptr())
-            return
-        if plan.tiles is not None:
-            tp = plan.tiles
-            if tp.JT == 16:
-                call("spr_exchange_gather_tiled_shape", tp.P, tp.JT, tp.R,
-                     B, T, tp.M, ptr(tp.tptr), ptr(tp.gstart),
-                     ptr(tp.mspan), ptr(tp.mlag), ptr(tp.mw), ptr(self.prev),
-                     ptr(self._gsum(plan)), stream_ptr())
-                return
-            call("spr_exchange_gather_tiled", tp.P, tp.JT, B, T, tp.M,
-                 ptr(tp.tptr), ptr(tp.gstart), ptr(tp.mrow),
-                 ptr(tp.mlag), ptr(tp.mw), ptr(self.prev),
-                 ptr(self._gsum(plan)), stream_ptr())
-            return
-        call("spr_exchange_gather_grouped", P, B, T, plan.Vd,
-             ptr(plan.row_ptr), ptr(plan.grp_ptr), ptr(plan.rowid),
-             ptr(plan.lag), ptr(plan.scal), ptr(self.prev),
-             ptr(self._gsum(plan)), stream_ptr())
+        gather_groups(plan, self.shape, self.prev, self._gsum(plan))
 
     def spread(self, plan):
         """Grouped plan, pass 2 of an order (directions; the running total

@@ -95,30 +95,58 @@ def _split_walls_into_patches(walls, patch_size):
 
 def directional_shoebox(seed=2, n_receivers=4, size=(10., 8., 3.),
                         patch_size=0.5, sh_order=7, n_bands=6,
-                        duration=1.5, rate=4000., max_order=30):
+                        duration=1.5, rate=4000., max_order=30,
+                        brdf="scattering"):
     """10x8x3 m shoebox at 0.5 m patches (1,072), one wall per patch with
     its own seeded BRDF (alpha in [0.02, 0.4], s in [0.1, 0.9] per band),
     64 directions, 6 octave bands, 4 receivers >= 1 m from walls, 4 kHz
-    bins over 1.5 s, order 30."""
+    bins over 1.5 s, order 30.
+
+    ``brdf="directional"``: the six walls unsplit (the same 1,072 patches
+    once subdivided), each with one BRDF from a seeded directional
+    scattering coefficient (``create_from_directional_scattering``, rows
+    uniform in [0.2, 1] normalised over the outgoing directions, alpha in
+    [0.02, 0.4]): rows of no diffuse + specular form, shared per wall;
+    ``materials`` then holds the walls' alpha and directional scattering."""
+    if brdf not in ("scattering", "directional"):
+        raise ValueError('brdf must be "scattering" or "directional"')
     rng = np.random.default_rng(seed)
     X, Y, Z = size
     dirs = hemisphere(sph_gaussian(sh_order=sh_order))
     freqs = OCTAVE_BANDS[1:1 + n_bands] if n_bands <= 7 else \
         OCTAVE_BANDS[:n_bands]
-    walls = _split_walls_into_patches(shoebox_room_stub(X, Y, Z), patch_size)
     groups, alphas, scats = [], [], []
-    for i in range(len(walls)):
+    split = _split_walls_into_patches(shoebox_room_stub(X, Y, Z), patch_size)
+    for i in range(len(split)):
         alpha = rng.uniform(0.02, 0.4, n_bands)
         s = rng.uniform(0.1, 0.9, n_bands)
-        groups.append(([i], _brdf(dirs, s, alpha, freqs)))
-        alphas.append(alpha)
-        scats.append(s)
+        if brdf == "scattering":
+            groups.append(([i], _brdf(dirs, s, alpha, freqs)))
+            alphas.append(alpha)
+            scats.append(s)
+    walls = split
+    if brdf == "directional":
+        # own generator: the draws above keep sources and receivers those
+        # of the scattering form
+        walls = shoebox_room_stub(X, Y, Z)
+        drng = np.random.default_rng(seed + 1000)
+        n = dirs.csize
+        for i in range(len(walls)):
+            alpha = drng.uniform(0.02, 0.4, n_bands)
+            sd = drng.uniform(0.2, 1.0, (n, n, n_bands))
+            sd /= sd.sum(axis=1, keepdims=True)
+            groups.append(([i], create_from_directional_scattering(
+                dirs, dirs, FrequencyData(sd, freqs),
+                FrequencyData(alpha.reshape(1, -1), freqs))))
+            alphas.append(alpha)
+            scats.append(sd)
     recv = np.stack([rng.uniform(1.0, X - 1.0, n_receivers),
                      rng.uniform(1.0, Y - 1.0, n_receivers),
                      rng.uniform(1.0, Z - 1.0, n_receivers)], axis=-1)
     src = np.array([[X * 0.3, Y * 0.4, Z * 0.5]])
     return Scene(
-        name=f"shoebox-{X:g}x{Y:g}x{Z:g}-p{patch_size:g}-dir",
+        name=f"shoebox-{X:g}x{Y:g}x{Z:g}-p{patch_size:g}-dir"
+             + ("-dirscat" if brdf == "directional" else ""),
         walls=walls, patch_size=patch_size, frequencies=freqs,
         brdf_groups=groups, directions=dirs,
         air_attenuation=air_attenuation_octaves(freqs), sources=src,

@@ -1,5 +1,6 @@
 """Host logic of the destination tilings, the directional-scattering canyon
-scene and the profile-name parsing of bench.py, on the CPU (no kernels)."""
+and shoebox scenes and the profile-name parsing of bench.py, on the CPU (no
+kernels)."""
 import os
 import sys
 
@@ -101,6 +102,39 @@ def test_directional_canyon_scene():
         assert np.ptp(rho[0, :, 0]) > 0
     with pytest.raises(ValueError, match="brdf"):
         scenes.street_canyon(brdf="lambert")
+
+
+def test_directional_shoebox_scene():
+    """directional_shoebox(brdf="directional"): the six walls unsplit (the
+    same 1,072 patches once subdivided), the same source and receivers as
+    the scattering form, and per wall a BRDF whose rows carry the wall's
+    1 - alpha as energy (brdf.py:133-224) and are not constant plus one
+    specular entry."""
+    from sparrowpy_amd import DirectionalRadiosityFast
+    a = scenes.directional_shoebox()
+    b = scenes.directional_shoebox(brdf="directional")
+    assert b.name == a.name + "-dirscat" and len(b.walls) == 6
+    np.testing.assert_array_equal(a.sources, b.sources)
+    np.testing.assert_array_equal(a.receivers, b.receivers)
+    ra = DirectionalRadiosityFast.from_polygon(a.walls, a.patch_size)
+    rb = DirectionalRadiosityFast.from_polygon(b.walls, b.patch_size)
+    assert ra.n_patches == rb.n_patches == 1072
+    dirs = b.directions
+    cos_o = dirs.cartesian[:, 2] / np.linalg.norm(dirs.cartesian, axis=1)
+    w = np.asarray(dirs.weights, float)
+    w = w * 2 * np.pi / w.sum()
+    alpha, sd = b.materials
+    assert alpha.shape == (6, 6) and sd.shape == (6, 64, 64, 6)
+    for n, (idx, br) in enumerate(b.brdf_groups):
+        assert idx == [n]
+        rho = np.asarray(br.freq)
+        energy = np.einsum("iob,o->ib", rho, cos_o * w)
+        np.testing.assert_allclose(
+            energy, np.broadcast_to(1 - alpha[n], energy.shape), rtol=1e-12)
+        row = rho[0, :, 0]
+        assert np.unique(np.round(row / row.max(), 12)).size > 2
+    with pytest.raises(ValueError, match="brdf"):
+        scenes.directional_shoebox(brdf="lambert")
 
 
 def test_profile_kernel_names():

@@ -110,3 +110,87 @@ def test_exchange_orders_sum_to_oracle_exchange():
                              r.distance_patches_to_source, r.pair_distances(),
                              r.fft, r.p2o, c, dt, K, r.visible_patches)
     np.testing.assert_allclose(sum(E), r.etc, rtol=1e-12, atol=1e-300)
+
+
+def directional_tables(dirs, n_walls, seed):
+    """Per wall, the freq array [S, D, 2] of a
+    create_from_directional_scattering BRDF (rows of no diffuse + specular
+    form) with random directional scattering and absorption."""
+    rng = np.random.default_rng(seed)
+    S = dirs.csize
+    out = []
+    for _ in range(n_walls):
+        sd = rng.uniform(.2, 1., (S, S, 2))
+        sd /= sd.sum(axis=1, keepdims=True)
+        alpha = rng.uniform(.05, .4, 2)
+        br = B.create_from_directional_scattering(
+            dirs, dirs, FrequencyData(sd, FREQS),
+            FrequencyData(alpha[None, :], FREQS))
+        out.append(np.asarray(br.freq, float))
+    return out
+
+
+def oracle_scene_general(walls, dirs, freqs, air=(2e-3, 5e-3)):
+    r = make_oracle(walls, .5)
+    for n, f in enumerate(freqs):
+        r.set_wall_brdf([n], f, dirs.cartesian, dirs.cartesian, FREQS)
+    r.set_air_attenuation(np.asarray(air), FREQS)
+    r.bake_geometry()
+    return r
+
+
+def test_oracle_table_gradient_matches_finite_differences():
+    """dL/dtable of general BRDF rows (directional scattering) against
+    central differences of single table entries (the stored table is the
+    BRDF times pi, so dL/dfreq = pi dL/dtable)."""
+    walls, dirs, ta, ts, alpha, scat, src, rcv, (T, c, dt, K) = _setup()
+    target = run(oracle_scene_general(walls, dirs,
+                                      directional_tables(dirs, len(walls), 3)),
+                 src, rcv, T, c, dt, K)
+    f0 = directional_tables(dirs, len(walls), 4)
+    r = oracle_scene_general(walls, dirs, f0)
+    L, gH = loss_and_grad(run(r, src, rcv, T, c, dt, K), target)
+    assert L > 0
+    gt = oadj.gradient(r, rcv, gH, K)
+    S = dirs.csize
+    assert gt.shape == (len(walls) * S, S, 2)
+    gmax = np.abs(gt).max() * np.pi
+    rng = np.random.default_rng(5)
+    h = 1e-5
+    checked = 0
+    for n, s, d, b in zip(rng.integers(0, len(walls), 12),
+                          rng.integers(0, S, 12), rng.integers(0, S, 12),
+                          rng.integers(0, 2, 12)):
+        vals = []
+        for sign in (1, -1):
+            f2 = [f.copy() for f in f0]
+            f2[n][s, d, b] += sign * h
+            Hs = run(oracle_scene_general(walls, dirs, f2), src, rcv, T, c,
+                     dt, K)
+            vals.append(loss_and_grad(Hs, target)[0])
+        fd = (vals[0] - vals[1]) / (2 * h)
+        an = np.pi * gt[n * S + s, d, b]
+        assert abs(fd - an) <= 1e-6 * gmax + 1e-5 * abs(fd), \
+            (n, s, d, b, fd, an)
+        checked += abs(an) > 1e-3 * gmax
+    assert checked >= 3   # not only entries without influence
+
+
+def test_directional_scattering_tables_match_brdf():
+    """adjoint.directional_scattering_tables (differentiable) equals
+    create_from_directional_scattering times pi, the stored table."""
+    import torch
+    from sparrowpy_amd import adjoint
+    _, dirs = small_room()
+    S = dirs.csize
+    rng = np.random.default_rng(7)
+    sd = rng.uniform(.2, 1., (2, S, S, 2))
+    sd /= sd.sum(axis=2, keepdims=True)
+    alpha = rng.uniform(.05, .4, (2, 2))
+    got = adjoint.directional_scattering_tables(
+        torch.tensor(alpha), torch.tensor(sd), dirs).numpy()
+    for n in range(2):
+        ref = np.asarray(B.create_from_directional_scattering(
+            dirs, dirs, FrequencyData(sd[n], FREQS),
+            FrequencyData(alpha[n][None, :], FREQS)).freq) * np.pi
+        np.testing.assert_allclose(got[n * S:(n + 1) * S], ref, rtol=1e-13)
