@@ -611,7 +611,10 @@ def main(argv=None):
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default="canyon",
                     choices=["canyon", "shoebox", "small", "batched",
-                             "inverse"])
+                             "inverse", "adjoint"],
+                    help="adjoint: only the adjoint/forward timings of the "
+                         "default run (diffuse + specular and general "
+                         "BRDF rows)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="target CPU time of the oracle baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -692,6 +695,18 @@ def main(argv=None):
             print(f"[bench] {msg}", file=sys.stderr, flush=True)
         fn = bench_batched if args.config == "batched" else bench_inverse
         print(json.dumps(fn(args, blog)), flush=True)
+        return 0
+    if args.config == "adjoint":
+        # the two adjoint timings of the default run, alone (to profile them)
+        def alog(msg):
+            print(f"[bench] {msg}", file=sys.stderr, flush=True)
+        ds, gen = adjoint_timing(alog), adjoint_brdf_timing(alog)
+        print(json.dumps({
+            "metric": "adjoint / forward time", "value": gen["adjoint_s"] /
+            gen["forward_s"], "unit": "ratio", "n_gpus": 1,
+            "higher_is_better": False, "data": "synthetic (seeded "
+            "directional shoebox)", "adjoint": ds, "adjoint_brdf": gen}),
+            flush=True)
         return 0
     sc = _scene(args.config, args.brdf)
     torch.cuda.init()
