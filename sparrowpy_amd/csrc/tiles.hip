@@ -372,15 +372,18 @@ int launch_tile_gather(int64_t P, int JT, int64_t B, int64_t T, int64_t M,
 // slot whose delay is within SMAX of the entry's largest delay (maxlag); a
 // slot with a larger spread reads its bins from global memory instead
 // (uniform branch, rare).  The batch's metadata goes to LDS beside the
-// windows: per (entry, slot) the LDS offset of the slot's view (maxlag -
-// delay, with the delay word's flags) and the weights, per entry its
-// source row and maxlag.  A wave reads its slot's 16 offset words with one
-// LDS load (one per lane), walks only the entries its slot has or ends a
-// group at (ballots over those words), takes each word with v_readlane
-// and reads each weight with a broadcast LDS load straight into the FMA's
-// operand, so a present entry costs the wave one v_readlane, one address
-// add, R LDS reads, R conversions and R FP64 FMAs; the next entry's reads
-// are issued before the current entry is summed.  Two LDS buffers (two
+// windows: per (entry, slot) the byte offset of the slot's view in the
+// window buffer with the flags absent / far / group end, and the weights;
+// per entry its source row and maxlag.  A wave reads its slot's 16 words
+// and 16 weights with one LDS load each (one per lane), walks only the
+// entries its slot has or ends a group at (ballots over the words), and
+// takes each entry's word and weight with v_readlane: a present entry
+// costs the wave three v_readlane, one mask and one add for its view's
+// address, R LDS reads, R conversions and R FP64 FMAs (the walk's scalar
+// work was as large as the sums' vector work when each entry built its
+// address and weight read from the delay word: 100.4 against 105.0 ms on
+// the street canyon); the next entry's reads are issued before the
+// current entry is summed.  Two LDS buffers (two
 // LDS objects): batch q+1's windows are copied from memory into LDS by the
 // loads themselves (no registers) while batch q is summed; each wave waits
 // for its copies (vmcnt) only before the one barrier per batch.  Each wave
@@ -414,9 +417,14 @@ struct WindowDma<N, N> {
                                              float *, uint32_t) {}
 };
 
+// flags of a staged word (its low 29 bits the view's byte offset)
+constexpr int32_t kAbsent = (int32_t)0x80000000u;
+constexpr int32_t kFar = 0x20000000;
+constexpr int32_t kAddrMask = 0x1fffffff;
+
 template <int NB, int JT, bool DS>
 struct BatchMeta {
-  int32_t word[NB][JT];       // (maxlag - delay) | flags; kLagMask | flags
+  int32_t word[NB][JT];       // view byte offset | kAbsent, kFar, group end
   double w[NB][JT];           // weights in the workgroup's band
   double dw[DS ? NB : 1][JT]; // diffuse weights (diffuse + specular plan)
   int32_t row[NB];            // source row
@@ -547,11 +555,21 @@ tile_lds_gather_kernel(
     m_max = reinterpret_cast<const int32_t *>(span)[mm * 4 + 1];
     m_w = wsrc[mm * JT + q_k];
   };
+  // the staged word of (entry e, slot kk): the byte offset of the slot's
+  // view in the window buffer, (e * W + min(maxlag - delay, SMAX)) * 4,
+  // with flags: kAbsent (slot without the entry; the offset then e * W * 4),
+  // kFar (delay more than SMAX below maxlag: the view is read from global
+  // memory) and kLastInGroup -- so an entry costs the walk one mask and
+  // one add to address its view
   auto meta_write = [&](int buf) {
     if (q_part == 0) {
       const int32_t l = m_word & kLagMask;
-      meta[buf].word[q_e][q_k] =
-          (l == kLagMask ? kLagMask : m_max - l) | (m_word & kLastInGroup);
+      const bool absent = l == kLagMask;
+      const int32_t off = absent ? 0 : m_max - l;
+      meta[buf].word[q_e][q_k] = ((q_e * W + min(off, SMAX)) * 4) |
+                                 (absent ? kAbsent : 0) |
+                                 (off > SMAX ? kFar : 0) |
+                                 (m_word & kLastInGroup);
     } else if (q_part == 1) {
       meta[buf].w[q_e][q_k] = m_w;
     } else if (DS && q_part == 2) {
@@ -577,14 +595,20 @@ tile_lds_gather_kernel(
     double w, dw;
     int32_t lw;
   };
-  auto read_entry = [&](int buf, int idx, int32_t words, Cur &c) {
+  // words, wts, dwts: lane e (mod NB) holds entry e's staged word and
+  // weights of this slot; an entry takes them with v_readlane
+  auto read_entry = [&](int buf, int idx, int32_t words, double wts,
+                        double dwts, Cur &c) {
     c.lw = __builtin_amdgcn_readlane(words, idx);
-    const int off = min(c.lw & kLagMask, SMAX);
-    const float *src = &win(buf)[idx][off + lane];
+    const float *src =
+        reinterpret_cast<const float *>(
+            reinterpret_cast<const char *>(&win(buf)[0][0]) +
+            (c.lw & kAddrMask)) +
+        lane;
 #pragma unroll
     for (int r = 0; r < R; ++r) c.v[r] = src[64 * r];
-    c.w = meta[buf].w[idx][k];
-    if (DS) c.dw = meta[buf].dw[DS ? idx : 0][k];
+    c.w = readlane_f64(wts, idx);
+    if (DS) c.dw = readlane_f64(dwts, idx);
   };
   auto sum_entry = [&](const float (&v)[R], double wk, double dk) {
     if (DS) {
@@ -599,16 +623,14 @@ tile_lds_gather_kernel(
       for (int r = 0; r < R; ++r) S[r] = fma(wk, (double)v[r], S[r]);
     }
   };
-  auto use_entry = [&](int buf, int idx, const Cur &c) {
-    const int32_t off = c.lw & kLagMask;
-    if (off <= SMAX) {
+  auto use_entry = [&](int buf, int mb, int idx, const Cur &c) {
+    if (!(c.lw & (kAbsent | kFar))) {
       sum_entry(c.v, c.w, c.dw);
-    } else if (off != kLagMask) {
+    } else if (!(c.lw & kAbsent)) {
       // delay far from the entry's others: its bins from global memory,
       // into registers of their own (the prefetched view stays untouched)
       const int32_t row = __builtin_amdgcn_readfirstlane(meta[buf].row[idx]);
-      const int32_t l =
-          __builtin_amdgcn_readfirstlane(meta[buf].maxlag[idx]) - off;
+      const int32_t l = lagw[(int64_t)(mb + idx) * JT + k] & kLagMask;
       const __amdgpu_buffer_rsrc_t rs = make_rsrc(
           prev + (int64_t)row * bstride + (int64_t)b * T, row_bytes);
       float v[R];
@@ -633,36 +655,40 @@ tile_lds_gather_kernel(
       g += 1;
     }
   };
-  auto compute = [&](int buf, int n_in) {
-    // lane e (mod NB) holds entry e's offset word of this slot
-    const int32_t words = meta[buf].word[lane & (NB - 1)][k];
+  auto compute = [&](int buf, int mb, int n_in) {
+    const int e = lane & (NB - 1);
+    const int32_t words = meta[buf].word[e][k];
+    const double wts = meta[buf].w[e][k];
+    const double dwts = DS ? meta[buf].dw[DS ? e : 0][k] : 0.0;
     const uint32_t in_mask = (uint32_t)((1ull << n_in) - 1ull);
     const uint32_t pres =
-        (uint32_t)__ballot((words & kLagMask) != kLagMask) & in_mask;
+        (uint32_t)__ballot((words & kAbsent) == 0) & in_mask;
     const uint32_t ends =
         (uint32_t)__ballot((words & kLastInGroup) != 0) & in_mask;
     uint32_t todo = __builtin_amdgcn_readfirstlane(pres | ends);
     const int n_todo = __builtin_popcount(todo);
     if (n_todo == 0) return;
     // i-th entry of the walk in ca (i even) or cb (i odd); the next one's
-    // reads are issued before the current one is summed (at the last, a
-    // harmless second read of the same entry)
+    // reads are issued before the current one is summed (after the last,
+    // a harmless read of entry NB - 1, whose staged word always addresses
+    // its own window)
+    constexpr uint32_t kStop = 1u << (NB - 1);
     Cur ca, cb;
     int idx = __builtin_ctz(todo);
-    read_entry(buf, idx, words, ca);
+    read_entry(buf, idx, words, wts, dwts, ca);
 #pragma unroll
     for (int i = 0; i < NB; i += 2) {
       if (i >= n_todo) break;  // uniform
       todo &= todo - 1u;
-      int nx = todo ? __builtin_ctz(todo) : idx;
-      read_entry(buf, nx, words, cb);
-      use_entry(buf, idx, ca);
+      int nx = __builtin_ctz(todo | kStop);
+      read_entry(buf, nx, words, wts, dwts, cb);
+      use_entry(buf, mb, idx, ca);
       idx = nx;
       if (i + 1 >= n_todo) break;
       todo &= todo - 1u;
-      nx = todo ? __builtin_ctz(todo) : idx;
-      read_entry(buf, nx, words, ca);
-      use_entry(buf, idx, cb);
+      nx = __builtin_ctz(todo | kStop);
+      read_entry(buf, nx, words, wts, dwts, ca);
+      use_entry(buf, mb, idx, cb);
       idx = nx;
     }
   };
@@ -685,7 +711,7 @@ tile_lds_gather_kernel(
       meta_load(mb + NB, n1);
       sp = span_of(mb + 2 * NB + k);
     }
-    if (active) compute(buf, n_at(mb));
+    if (active) compute(buf, mb, n_at(mb));
     if (n1 > 0) meta_write(buf ^ 1);
     __builtin_amdgcn_s_waitcnt(0x0f70);  // vmcnt(0): this wave's window copies
     __syncthreads();
