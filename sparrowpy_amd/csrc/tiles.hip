@@ -375,15 +375,15 @@ int launch_tile_gather(int64_t P, int JT, int64_t B, int64_t T, int64_t M,
 // windows: per (entry, slot) the byte offset of the slot's view in the
 // window buffer with the flags absent / far / group end, and the weights;
 // per entry its source row and maxlag.  A wave reads its slot's 16 words
-// and 16 weights with one LDS load each (one per lane), walks only the
-// entries its slot has or ends a group at (ballots over the words), and
-// takes each entry's word and weight with v_readlane: a present entry
-// costs the wave three v_readlane, one mask and one add for its view's
-// address, R LDS reads, R conversions and R FP64 FMAs (the walk's scalar
-// work was as large as the sums' vector work when each entry built its
-// address and weight read from the delay word: 100.4 against 105.0 ms on
-// the street canyon); the next entry's reads are issued before the
-// current entry is summed.  Two LDS buffers (two
+// with one LDS load (one per lane), walks only the entries its slot has or
+// ends a group at (ballots over the words), takes each entry's word with
+// v_readlane and reads its weight with a broadcast LDS load: a present
+// entry costs the wave one v_readlane, one mask and one add for its view's
+// address, R + 1 LDS reads, R conversions and R FP64 FMAs (the walk's
+// scalar work was as large as the sums' vector work when each entry built
+// its view's address from the delay word: 100.4 against 105.0 ms on the
+// street canyon); the next entry's reads are issued before the current
+// entry is summed.  Two LDS buffers (two
 // LDS objects): batch q+1's windows are copied from memory into LDS by the
 // loads themselves (no registers) while batch q is summed; each wave waits
 // for its copies (vmcnt) only before the one barrier per batch.  Each wave
@@ -424,9 +424,11 @@ constexpr int32_t kAddrMask = 0x1fffffff;
 
 template <int NB, int JT, bool DS>
 struct BatchMeta {
-  int32_t word[NB][JT];       // view byte offset | kAbsent, kFar, group end
-  double w[NB][JT];           // weights in the workgroup's band
-  double dw[DS ? NB : 1][JT]; // diffuse weights (diffuse + specular plan)
+  // rows padded to JT + 1: a wave reads one slot's column (16 entries) per
+  // array, which at a row stride of JT words would fall on a few LDS banks
+  int32_t word[NB][JT + 1];       // view byte offset | kAbsent, kFar, end
+  double w[NB][JT + 1];           // weights in the workgroup's band
+  double dw[DS ? NB : 1][JT + 1]; // diffuse weights (diffuse + specular)
   int32_t row[NB];            // source row
   int32_t maxlag[NB];         // largest delay of the entry's present slots
 };
@@ -595,10 +597,10 @@ tile_lds_gather_kernel(
     double w, dw;
     int32_t lw;
   };
-  // words, wts, dwts: lane e (mod NB) holds entry e's staged word and
-  // weights of this slot; an entry takes them with v_readlane
-  auto read_entry = [&](int buf, int idx, int32_t words, double wts,
-                        double dwts, Cur &c) {
+  // words: lane e (mod NB) holds entry e's staged word of this slot, taken
+  // with v_readlane; the weights are read by broadcast LDS loads (with
+  // v_readlane from a per-batch lane load instead: 94.8 against 93.3 ms)
+  auto read_entry = [&](int buf, int idx, int32_t words, Cur &c) {
     c.lw = __builtin_amdgcn_readlane(words, idx);
     const float *src =
         reinterpret_cast<const float *>(
@@ -607,8 +609,8 @@ tile_lds_gather_kernel(
         lane;
 #pragma unroll
     for (int r = 0; r < R; ++r) c.v[r] = src[64 * r];
-    c.w = readlane_f64(wts, idx);
-    if (DS) c.dw = readlane_f64(dwts, idx);
+    c.w = meta[buf].w[idx][k];
+    if (DS) c.dw = meta[buf].dw[DS ? idx : 0][k];
   };
   auto sum_entry = [&](const float (&v)[R], double wk, double dk) {
     if (DS) {
@@ -656,10 +658,7 @@ tile_lds_gather_kernel(
     }
   };
   auto compute = [&](int buf, int mb, int n_in) {
-    const int e = lane & (NB - 1);
-    const int32_t words = meta[buf].word[e][k];
-    const double wts = meta[buf].w[e][k];
-    const double dwts = DS ? meta[buf].dw[DS ? e : 0][k] : 0.0;
+    const int32_t words = meta[buf].word[lane & (NB - 1)][k];
     const uint32_t in_mask = (uint32_t)((1ull << n_in) - 1ull);
     const uint32_t pres =
         (uint32_t)__ballot((words & kAbsent) == 0) & in_mask;
@@ -675,19 +674,19 @@ tile_lds_gather_kernel(
     constexpr uint32_t kStop = 1u << (NB - 1);
     Cur ca, cb;
     int idx = __builtin_ctz(todo);
-    read_entry(buf, idx, words, wts, dwts, ca);
+    read_entry(buf, idx, words, ca);
 #pragma unroll
     for (int i = 0; i < NB; i += 2) {
       if (i >= n_todo) break;  // uniform
       todo &= todo - 1u;
       int nx = __builtin_ctz(todo | kStop);
-      read_entry(buf, nx, words, wts, dwts, cb);
+      read_entry(buf, nx, words, cb);
       use_entry(buf, mb, idx, ca);
       idx = nx;
       if (i + 1 >= n_todo) break;
       todo &= todo - 1u;
       nx = __builtin_ctz(todo | kStop);
-      read_entry(buf, nx, words, wts, dwts, ca);
+      read_entry(buf, nx, words, ca);
       use_entry(buf, mb, idx, cb);
       idx = nx;
     }
