@@ -3,19 +3,27 @@
 The reference computes no gradients; this module differentiates its
 forward path -- injection (``RadiosityFast.py:988-1034``), the energy
 exchange (``:1073-1145``) and the mono receiver collection (``:570-602``,
-``:1148-1185``) -- for scenes whose BRDF rows are a diffuse constant plus
-one specular entry (what ``create_from_scattering``, ``brdf.py:8-130``,
-produces).  Such a row is ``A[k, b] * 1 + C[k, b] * e_m(k)``; the forward
-runs on the diffuse + specular exchange plan with weights rebuilt from
-``(A, C)``, and the backward is the hand-written adjoint
-(``spr_exchange_adjoint_ds``): the transposed exchange, order by order from
-the last, with the forward order histograms recomputed from checkpoints
-kept every ``checkpoint_every`` orders.
+``:1148-1185``).  Two forms:
 
-:class:`ExchangeAdjoint` holds one scene/source/receiver set;
-:func:`receiver_histograms` is its ``torch.autograd.Function`` entry and
-:func:`scattering_tables` builds ``(A, C)`` from absorption and scattering
-coefficients differentiably, so gradients reach those coefficients.
+* BRDF rows that are a diffuse constant plus one specular entry (what
+  ``create_from_scattering``, ``brdf.py:8-130``, produces): such a row is
+  ``A[k, b] * 1 + C[k, b] * e_m(k)``; the forward runs on the diffuse +
+  specular exchange plan with weights rebuilt from ``(A, C)``, and the
+  backward is the hand-written adjoint (``spr_exchange_adjoint_ds``).
+  :class:`ExchangeAdjoint` holds one scene/source/receiver set;
+  :func:`receiver_histograms` is its ``torch.autograd.Function`` entry and
+  :func:`scattering_tables` builds ``(A, C)`` from absorption and
+  scattering coefficients differentiably.
+* BRDF rows of any form (``create_from_directional_scattering``,
+  ``brdf.py:133-224``, measured tables): :class:`BRDFExchangeAdjoint` on
+  the grouped plan, gradients with respect to the whole stored table;
+  :func:`receiver_histograms_brdf` is its entry and
+  :func:`directional_scattering_tables` builds the table from absorption
+  and directional scattering coefficients differentiably.
+
+Both run the transposed exchange order by order from the last, with the
+forward order histograms recomputed from checkpoints kept every
+``checkpoint_every`` orders.
 """
 from __future__ import annotations
 
