@@ -337,65 +337,91 @@ __global__ __launch_bounds__(256) void grouped_spread_adjoint_kernel(
 
 // gg[g,b,d] += sum_t mu[j_g,d,b,t] gsum[g,b,t] for the groups g of
 // destination j: per (j, band) the product of the group sums [G_j x T]
-// with mu[j,:,b,:]^T [T x D], in tiles of 64 groups x 64 directions, the
-// bins in chunks of 32 staged in LDS (float32: staged as float64 the
-// launch took 16.2 ms against 15.2 on the directional shoebox).  Each lane
-// sums 4 x 4 (group, direction) pairs in float64 and owns them: no
-// atomics, and the result does not depend on the launch's timing.
+// with mu[j,:,b,:]^T [T x D], in tiles of 16 NQ groups x 16 NR directions
+// (NQ from the groups left, NR from D: no tile rows beyond them; with
+// 64 x 64 tiles throughout the directional shoebox's launch took 15.3 ms),
+// the bins in chunks of 32 staged in LDS (float32: staged as float64 the
+// launch took 16.2 ms against 15.2).  Each lane sums NQ x NR (group,
+// direction) pairs in float64 and owns them: no atomics, and the result
+// does not depend on the launch's timing.
+constexpr int kTgChunk = 32;
+
+template <int NQ, int NR>
+__device__ __forceinline__ void table_grad_tile(
+    int D, int B, int T, int DP, int b, int64_t gt, int64_t g1,
+    const float *__restrict__ mrow, const float *__restrict__ gsum,
+    double *__restrict__ gg, float (&sg)[64][kTgChunk + 1],
+    float (&sm)[64][kTgChunk + 1]) {
+  constexpr int TC = kTgChunk;
+  const int tid = (int)threadIdx.x;
+  const int tg = tid >> 4, td = tid & 15;
+  const int64_t dstride = (int64_t)B * T;
+  double acc[NQ][NR];
+#pragma unroll
+  for (int q = 0; q < NQ; ++q)
+#pragma unroll
+    for (int r = 0; r < NR; ++r) acc[q][r] = 0.0;
+  for (int t0 = 0; t0 < T; t0 += TC) {
+    // 256 lanes stage 8 rows of 32 bins per step
+    const int col = tid % TC, t = t0 + col;
+#pragma unroll
+    for (int s = 0; s < 2 * NQ; ++s) {
+      const int row = tid / TC + 8 * s;
+      const int64_t g = gt + row;
+      sg[row][col] =
+          (g < g1 && t < T) ? gsum[(g * B + b) * (int64_t)T + t] : 0.f;
+    }
+#pragma unroll
+    for (int s = 0; s < 2 * NR; ++s) {
+      const int d = tid / TC + 8 * s;
+      sm[d][col] = (d < D && t < T) ? mrow[d * dstride + t] : 0.f;
+    }
+    __syncthreads();
+#pragma unroll 2
+    for (int c = 0; c < TC; ++c) {
+      double a[NQ], m[NR];
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) a[q] = (double)sg[tg + 16 * q][c];
+#pragma unroll
+      for (int r = 0; r < NR; ++r) m[r] = (double)sm[td + 16 * r][c];
+#pragma unroll
+      for (int q = 0; q < NQ; ++q)
+#pragma unroll
+        for (int r = 0; r < NR; ++r) acc[q][r] = fma(a[q], m[r], acc[q][r]);
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int q = 0; q < NQ; ++q)
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+      const int64_t g = gt + tg + 16 * q;
+      const int d = td + 16 * r;
+      if (g < g1 && d < D) gg[(g * B + b) * DP + d] += acc[q][r];
+    }
+}
+
+template <int NR>
 __global__ __launch_bounds__(256) void grouped_table_grad_kernel(
     int D, int B, int T, int DP, const int64_t *__restrict__ grp_ptr,
     const float *__restrict__ mu, const float *__restrict__ gsum,
     double *__restrict__ gg) {
-  constexpr int TC = 32;
-  __shared__ float sg[64][TC + 1];
-  __shared__ float sm[64][TC + 1];
+  __shared__ float sg[64][kTgChunk + 1];
+  __shared__ float sm[64][kTgChunk + 1];
   const int b = (int)(blockIdx.x % B);
   const int64_t j = blockIdx.x / B;
   const int64_t g0 = grp_ptr[j], g1 = grp_ptr[j + 1];
-  const int tid = (int)threadIdx.x;
-  const int tg = tid >> 4, td = tid & 15;
-  const int64_t dstride = (int64_t)B * T;
   const float *mrow = mu + (j * D * B + b) * (int64_t)T;  // (j, 0, b, 0)
   for (int64_t gt = g0; gt < g1; gt += 64) {
-    double acc[4][4];
-#pragma unroll
-    for (int q = 0; q < 4; ++q)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) acc[q][r] = 0.0;
-    for (int t0 = 0; t0 < T; t0 += TC) {
-#pragma unroll
-      for (int s = 0; s < 64 * TC / 256; ++s) {
-        const int idx = tid + 256 * s;
-        const int row = idx / TC, col = idx % TC;
-        const int t = t0 + col;
-        const int64_t g = gt + row;
-        sg[row][col] =
-            (g < g1 && t < T) ? gsum[(g * B + b) * (int64_t)T + t] : 0.f;
-        sm[row][col] = (row < D && t < T) ? mrow[row * dstride + t] : 0.f;
-      }
-      __syncthreads();
-#pragma unroll 4
-      for (int c = 0; c < TC; ++c) {
-        double a[4], m[4];
-#pragma unroll
-        for (int q = 0; q < 4; ++q) a[q] = (double)sg[tg + 16 * q][c];
-#pragma unroll
-        for (int r = 0; r < 4; ++r) m[r] = (double)sm[td + 16 * r][c];
-#pragma unroll
-        for (int q = 0; q < 4; ++q)
-#pragma unroll
-          for (int r = 0; r < 4; ++r) acc[q][r] = fma(a[q], m[r], acc[q][r]);
-      }
-      __syncthreads();
-    }
-#pragma unroll
-    for (int q = 0; q < 4; ++q)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int64_t g = gt + tg + 16 * q;
-        const int d = td + 16 * r;
-        if (g < g1 && d < D) gg[(g * B + b) * DP + d] += acc[q][r];
-      }
+    const int64_t left = g1 - gt;  // uniform
+    if (left > 48)
+      table_grad_tile<4, NR>(D, B, T, DP, b, gt, g1, mrow, gsum, gg, sg, sm);
+    else if (left > 32)
+      table_grad_tile<3, NR>(D, B, T, DP, b, gt, g1, mrow, gsum, gg, sg, sm);
+    else if (left > 16)
+      table_grad_tile<2, NR>(D, B, T, DP, b, gt, g1, mrow, gsum, gg, sg, sm);
+    else
+      table_grad_tile<1, NR>(D, B, T, DP, b, gt, g1, mrow, gsum, gg, sg, sm);
   }
 }
 
@@ -674,9 +700,20 @@ extern "C" int spr_grouped_table_grad(int64_t P, int64_t D, int64_t B,
               "spr_grouped_table_grad: null pointer");
   SPR_REQUIRE(P * B < (int64_t)0x7fffffff,
               "spr_grouped_table_grad: grid too large");
-  hipLaunchKernelGGL(grouped_table_grad_kernel, dim3((unsigned)(P * B)),
-                     dim3(256), 0, as_stream(stream), (int)D, (int)B, (int)T,
-                     (int)DP, grp_ptr, mu, gsum, gg);
+  hipStream_t s = as_stream(stream);
+#define SPR_TG(NRV)                                                           \
+  hipLaunchKernelGGL((grouped_table_grad_kernel<NRV>),                        \
+                     dim3((unsigned)(P * B)), dim3(256), 0, s, (int)D,        \
+                     (int)B, (int)T, (int)DP, grp_ptr, mu, gsum, gg)
+  if (D <= 16)
+    SPR_TG(1);
+  else if (D <= 32)
+    SPR_TG(2);
+  else if (D <= 48)
+    SPR_TG(3);
+  else
+    SPR_TG(4);
+#undef SPR_TG
   SPR_LAUNCH_CHECK();
   return SPR_OK;
 }
