@@ -543,8 +543,10 @@ tile_lds_gather_kernel(
   // weight array chosen per wave), without branches: the values are only
   // used after the sums, so nothing waits for them (or, vmcnt being in
   // order, for the window copies issued before them) in between.  Pairs
-  // past the batch's end are staged with clamped values the sums never
-  // read.
+  // past the batch's end are staged with the last entry's values (their
+  // own entry index in the offset): the sums never use them, and the
+  // walk's final prefetch, which may read entry NB - 1, stays inside that
+  // entry's window.
   const int q_part = k / (NM / 64);
   const int q = (int)threadIdx.x - q_part * NM;
   const int q_e = q / JT, q_k = q % JT;
