@@ -482,8 +482,13 @@ int64_t spr_energy_exchange_workspace(int64_t P, int64_t D, int64_t B,
  * argument list: T, e0dir[P,D,B], d0[P], dist[P,P], fft[P,P,D,B],
  * p2o[P,P] (int64), c, dt, K, pairs[V,2] (int32) -> etc[P,D,B,T]
  * (float32).  workspace: spr_energy_exchange_workspace(...) bytes.
- * Synchronous: returns SPR_ERR_RANGE if a source delay is >= T (the
- * reference raises IndexError there). */
+ * When each destination's F~ rows are, band by band, multiples of a few
+ * vectors (entries that share a BRDF row; checked entry by entry to 2^-28
+ * of the row's largest element) in at most min(ceil(V/2), 2*P*D) groups, it
+ * runs the grouped two-pass exchange (group sums, then a spread by the
+ * group's row), else the per-pair plan; spr_energy_exchange_groups tells
+ * which.  Synchronous: returns SPR_ERR_RANGE if a source delay is >= T
+ * (the reference raises IndexError there). */
 int spr_energy_exchange(int64_t T, int64_t P, int64_t D, int64_t B,
                         const double *e0dir, const double *d0,
                         const double *dist, const double *fft,
@@ -492,9 +497,14 @@ int spr_energy_exchange(int64_t T, int64_t P, int64_t D, int64_t B,
                         void *workspace, int64_t workspace_bytes,
                         void *stream);
 
+/* Groups of the grouped plan the last spr_energy_exchange(_noalloc) call
+ * on this thread ran, 0 if it ran the per-pair plan.
+ * Diagnostic of spr_energy_exchange; no counterpart in the reference. */
+int64_t spr_energy_exchange_groups(void);
+
 /* Scratch built into the library for spr_energy_exchange_noalloc, per GPU
  * (64 MiB: enough for the 148-patch shoebox with 16 directions, one band and
- * 1,000 bins, which needs about 31 MB). */
+ * 1,000 bins, which needs about 49 MB). */
 #define SPR_NOALLOC_SCRATCH_BYTES (64ll << 20)
 
 /* No counterpart in the reference.  spr_energy_exchange for callers that

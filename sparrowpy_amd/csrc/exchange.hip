@@ -935,6 +935,28 @@ __global__ __launch_bounds__(256) void group_key_kernel(
   }
 }
 
+// Ascending bitonic sort of k[0, np) in LDS by the whole workgroup; np is a
+// power of two, the same for every thread, and k is filled and
+// synchronised before the call.
+__device__ void lds_bitonic_sort(uint64_t *k, int np) {
+  for (int size = 2; size <= np; size <<= 1) {
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      for (int q = threadIdx.x; q < np; q += blockDim.x) {
+        int p = q ^ stride;
+        if (p > q) {
+          bool up = (q & size) == 0;
+          uint64_t a = k[q], c = k[p];
+          if ((a > c) == up) {
+            k[q] = c;
+            k[p] = a;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+}
+
 // Sort each destination row by (key, position) with a bitonic sort in LDS
 // (stable: ties keep ascending source order) and count its groups.
 // Entries whose key is none_key form no group; a non-empty row gets
@@ -961,22 +983,7 @@ __global__ __launch_bounds__(1024) void group_sort_kernel(
   }
   if (threadIdx.x == 0) groups = 0;
   __syncthreads();
-  for (int size = 2; size <= npad; size <<= 1) {
-    for (int stride = size >> 1; stride > 0; stride >>= 1) {
-      for (int q = threadIdx.x; q < npad; q += blockDim.x) {
-        int p = q ^ stride;
-        if (p > q) {
-          bool up = (q & size) == 0;
-          uint64_t a = k[q], c = k[p];
-          if ((a > c) == up) {
-            k[q] = c;
-            k[p] = a;
-          }
-        }
-      }
-      __syncthreads();
-    }
-  }
+  lds_bitonic_sort(k, npad);
   int local = 0;
   for (int q = threadIdx.x; q < n; q += blockDim.x) {
     const uint32_t kq = (uint32_t)(k[q] >> 32);
@@ -1062,6 +1069,171 @@ __global__ void rtab_kernel(int64_t NK, int64_t D, int64_t B, int64_t DP,
   double v = 0.0;
   if (d < D) v = brdf ? brdf[(k * D + d) * B + b] : 1.0;
   rtab[q] = v;
+}
+
+// ---------------------------------------------------------------------
+// grouped plan from the dense F~ (spr_energy_exchange)
+// ---------------------------------------------------------------------
+// The F~ rows of a destination's entries that share a BRDF row are, band by
+// band, multiples of one vector: F~[i, j, :, b] = s_ijb * rho_b.  The rows
+// are matched by a hash of their band-wise normalised values (each band
+// divided by its first element of largest magnitude, the pivot, and
+// rounded to multiples of 2^-30), sorted by it within the destination, and
+// every entry is then checked against its group's first entry:
+//   |F~_e[d, b] - s_eb F~_first[d, b]| <= 2^-28 max_d |F~_e[d, b]|
+// with s_eb = F~_e[piv, b] / F~_first[piv, b] at the first entry's pivot.
+// A failed check (a hash collision, or values that are not numbers) sends
+// the whole call to the per-pair plan.
+constexpr double kRowQuantum = 1073741824.0;                  // 2^30
+constexpr double kRowTol = 3.7252902984619140625e-09;         // 2^-28
+constexpr uint64_t kHashPosMask = (uint64_t)kSortMax - 1;     // 13 bits
+static_assert((kSortMax & (kSortMax - 1)) == 0, "kSortMax: a power of 2");
+
+// splitmix64's finaliser (Steele, Lea and Flood, OOPSLA 2014) on h + x.
+__device__ __forceinline__ uint64_t mix64(uint64_t h, uint64_t x) {
+  uint64_t z = h + x + 0x9e3779b97f4a7c15ull;
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+  return z ^ (z >> 31);
+}
+
+// Index of the first element of largest magnitude of f[0], f[stride], ...,
+// f[(n-1) stride]; -1 when none is nonzero (all 0, or not numbers).
+__device__ __forceinline__ int row_pivot(const double *__restrict__ f,
+                                         int64_t n, int64_t stride) {
+  int piv = -1;
+  double m = 0.0;
+  for (int64_t d = 0; d < n; ++d) {
+    const double a = fabs(f[d * stride]);
+    if (a > m) {
+      m = a;
+      piv = (int)d;
+    }
+  }
+  return piv;
+}
+
+// hkey[e] = hash of entry e's normalised F~ row (all bands).
+__global__ __launch_bounds__(256) void dense_row_hash_kernel(
+    int64_t P, int64_t D, int64_t B, const int64_t *__restrict__ row_ptr,
+    const int32_t *__restrict__ src, const double *__restrict__ fft,
+    uint64_t *__restrict__ hkey) {
+  const int64_t j = blockIdx.x;
+  for (int64_t e = row_ptr[j] + threadIdx.x; e < row_ptr[j + 1];
+       e += blockDim.x) {
+    const double *f = fft + ((int64_t)src[e] * P + j) * D * B;
+    uint64_t h = 0x243f6a8885a308d3ull;
+    for (int64_t b = 0; b < B; ++b) {
+      const int piv = row_pivot(f + b, D, B);
+      h = mix64(h, (uint64_t)(int64_t)piv);
+      if (piv < 0) continue;
+      const double p = f[piv * B + b];
+      for (int64_t d = 0; d < D; ++d)
+        h = mix64(h, (uint64_t)__double2ll_rn(f[d * B + b] / p * kRowQuantum));
+    }
+    hkey[e] = h;
+  }
+}
+
+// Sort each destination row by (hash, position) in LDS -- the position in
+// the hash's low 13 bits, so ties keep ascending source order -- and mark
+// the group starts: start[e] = 1 where the rest of the hash changes.
+// A row longer than kSortMax sets *bad.  LDS: npad >= every row's length
+// (capped at kSortMax) keys, source ids and a group count.
+__global__ __launch_bounds__(1024) void dense_hash_sort_kernel(
+    int64_t P, const int64_t *__restrict__ row_ptr, int32_t *__restrict__ src,
+    const uint64_t *__restrict__ hkey, int32_t *__restrict__ start,
+    int64_t *__restrict__ grp_count, int32_t *__restrict__ bad) {
+  extern __shared__ uint64_t smem64[];
+  const int64_t j = blockIdx.x;
+  const int64_t e0 = row_ptr[j];
+  const int64_t len = row_ptr[j + 1] - e0;
+  if (len > kSortMax) {  // uniform
+    if (threadIdx.x == 0) {
+      grp_count[j] = 0;
+      atomicOr(bad, 1);
+    }
+    return;
+  }
+  const int n = (int)len;
+  int np = 1;
+  while (np < n) np <<= 1;
+  uint64_t *k = smem64;
+  int32_t *sv = reinterpret_cast<int32_t *>(k + np);
+  int &groups = sv[np];
+  for (int q = threadIdx.x; q < np; q += blockDim.x) {
+    if (q < n) {
+      k[q] = (hkey[e0 + q] & ~kHashPosMask) | (uint64_t)q;
+      sv[q] = src[e0 + q];
+    } else {
+      k[q] = ~0ull;
+    }
+  }
+  if (threadIdx.x == 0) groups = 0;
+  __syncthreads();
+  lds_bitonic_sort(k, np);
+  int local = 0;
+  for (int q = threadIdx.x; q < n; q += blockDim.x) {
+    src[e0 + q] = sv[k[q] & kHashPosMask];
+    const bool st =
+        q == 0 || (k[q - 1] & ~kHashPosMask) != (k[q] & ~kHashPosMask);
+    start[e0 + q] = st ? 1 : 0;
+    local += st ? 1 : 0;
+  }
+  atomicAdd(&groups, local);
+  __syncthreads();
+  if (threadIdx.x == 0) grp_count[j] = groups;
+}
+
+// The grouped plan on the sorted rows, one wavefront per destination row:
+// group keys (the entry index of each group's first entry, whose F~ row in
+// coef is the group's table row), source rows, last-in-group flags and
+// the per-band scalars scal[b * Vd + e] = s_eb; an entry that is not s_e
+// times its group's first row sets *bad.
+__global__ __launch_bounds__(256) void dense_group_fill_kernel(
+    int64_t P, int64_t D, int64_t B, int64_t DP, int64_t Vd,
+    const int64_t *__restrict__ row_ptr, const int64_t *__restrict__ grp_ptr,
+    const int32_t *__restrict__ src, const int32_t *__restrict__ start,
+    const int32_t *__restrict__ dir, const double *__restrict__ coef,
+    int32_t *__restrict__ grp_key, int32_t *__restrict__ rowid,
+    int32_t *__restrict__ lag, double *__restrict__ scal,
+    int32_t *__restrict__ bad) {
+  const int lane = threadIdx.x & 63;
+  const int64_t j = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (j >= P) return;
+  const int64_t e0 = row_ptr[j], e1 = row_ptr[j + 1];
+  int64_t g = grp_ptr[j] - 1;  // group of the last entry before the chunk
+  int64_t first = e0;          // and its first entry
+  bool ok = true;
+  const uint64_t upto = lane == 63 ? ~0ull : (2ull << lane) - 1ull;
+  for (int64_t base = e0; base < e1; base += 64) {
+    const int64_t e = base + lane;
+    const bool in = e < e1;
+    const bool st = in && start[e] != 0;
+    const uint64_t m = __ballot(st);
+    const uint64_t ml = m & upto;
+    const int64_t ge = g + __popcll(ml);
+    const int64_t fe = ml ? base + 63 - __clzll(ml) : first;
+    if (m) first = base + 63 - __clzll(m);
+    g += __popcll(m);
+    if (!in) continue;
+    if (st) grp_key[ge] = (int32_t)e;
+    const bool last = e + 1 == e1 || start[e + 1] != 0;
+    lag[e] |= last ? kLastInGroup : 0;
+    rowid[e] = (int32_t)((int64_t)src[e] * D + dir[e]);
+    for (int64_t b = 0; b < B; ++b) {
+      const double *r = coef + (fe * B + b) * DP;
+      const double *f = coef + (e * B + b) * DP;
+      const int piv = row_pivot(r, D, 1);
+      const double s = piv >= 0 ? f[piv] / r[piv] : 0.0;
+      double mx = 0.0;
+      for (int64_t d = 0; d < D; ++d) mx = fmax(mx, fabs(f[d]));
+      for (int64_t d = 0; d < D; ++d)
+        if (!(fabs(f[d] - s * r[d]) <= kRowTol * mx)) ok = false;
+      scal[b * Vd + e] = s;
+    }
+  }
+  if (!ok) atomicOr(bad, 1);
 }
 
 // Diffuse + specular decomposition of BRDF rows: row k (brdf[k, :, b]) is
@@ -1747,8 +1919,16 @@ extern "C" int spr_exchange_step_grouped(
 }
 
 namespace {
+// Workspace of spr_energy_exchange.  Besides the per-pair plan and the
+// three state buffers it holds the grouped plan built from the dense F~
+// (hashes alias coef, which is filled after the sort) with group sums for
+// at most gcap = min(ceil(Vd / 4), 2 P D) groups: beyond a quarter of the
+// entries the per-pair plan is used, and the group sums never take more
+// than two state buffers.
 struct DenseLayout {
-  size_t vis, row_ptr, src, lag, dir, coef, prev, cur, lo, err, total;
+  size_t vis, row_ptr, src, lag, dir, coef, prev, cur, lo, err, start, rowid,
+      grp_ptr, grp_key, scal, gsum, bad, total;
+  int64_t gcap;
 };
 
 DenseLayout dense_layout(int64_t P, int64_t D, int64_t B, int64_t T,
@@ -1756,31 +1936,37 @@ DenseLayout dense_layout(int64_t P, int64_t D, int64_t B, int64_t T,
   DenseLayout L;
   int64_t DP = spr_padded_directions(D);
   int64_t Vd = 2 * V;
+  L.gcap = std::min((Vd + 3) / 4, 2 * P * D);
   size_t n_state = (size_t)P * D * B * T;
   size_t off = 0;
-  L.vis = off;
-  off = align256(off + (size_t)P * P);
-  L.row_ptr = off;
-  off = align256(off + sizeof(int64_t) * (P + 1));
-  L.src = off;
-  off = align256(off + sizeof(int32_t) * Vd);
-  L.lag = off;
-  off = align256(off + sizeof(int32_t) * Vd);
-  L.dir = off;
-  off = align256(off + sizeof(int32_t) * Vd);
-  L.coef = off;
-  off = align256(off + sizeof(double) * Vd * B * (DP > 0 ? DP : 1));
-  L.prev = off;
-  off = align256(off + sizeof(float) * n_state);
-  L.cur = off;
-  off = align256(off + sizeof(float) * n_state);
-  L.lo = off;
-  off = align256(off + sizeof(float) * n_state);
-  L.err = off;
-  off = align256(off + sizeof(int));
+  auto take = [&](size_t &at, size_t bytes) {
+    at = off;
+    off = align256(off + bytes);
+  };
+  take(L.vis, (size_t)P * P);
+  take(L.row_ptr, sizeof(int64_t) * (P + 1));
+  take(L.src, sizeof(int32_t) * Vd);
+  take(L.lag, sizeof(int32_t) * Vd);
+  take(L.dir, sizeof(int32_t) * Vd);
+  take(L.coef, sizeof(double) * Vd * B * (DP > 0 ? DP : 1));
+  take(L.prev, sizeof(float) * n_state);
+  take(L.cur, sizeof(float) * n_state);
+  take(L.lo, sizeof(float) * n_state);
+  take(L.err, sizeof(int));
+  take(L.start, sizeof(int32_t) * Vd);
+  take(L.rowid, sizeof(int32_t) * Vd);
+  take(L.grp_ptr, sizeof(int64_t) * (P + 1));
+  take(L.grp_key, sizeof(int32_t) * L.gcap);
+  take(L.scal, sizeof(double) * B * Vd);
+  take(L.gsum, sizeof(float) * L.gcap * B * T);
+  take(L.bad, sizeof(int32_t));
   L.total = off;
   return L;
 }
+
+// Groups of the last spr_energy_exchange call on this thread (0: the
+// per-pair plan ran).
+thread_local int64_t last_dense_groups = 0;
 }  // namespace
 
 extern "C" int64_t spr_energy_exchange_workspace(int64_t P, int64_t D,
@@ -1790,11 +1976,82 @@ extern "C" int64_t spr_energy_exchange_workspace(int64_t P, int64_t D,
   return (int64_t)dense_layout(P, D, B, T, V).total;
 }
 
+extern "C" int64_t spr_energy_exchange_groups(void) {
+  return last_dense_groups;
+}
+
 namespace {
+// The grouped plan from the dense F~ on the destination rows (row_ptr,
+// src) already in the workspace: *groups = G on success, 0 (with src
+// possibly reordered) when the rows do not factorise into at most gcap
+// groups, a row is longer than the LDS sort takes, or the sizes are beyond
+// the grouped kernels' limits.
+int dense_grouped_plan(int64_t T, int64_t P, int64_t D, int64_t B,
+                       const double *dist, const double *fft,
+                       const int64_t *p2o, double c, double dt, int64_t V,
+                       char *ws, const DenseLayout &L, hipStream_t s,
+                       int64_t *groups) {
+  *groups = 0;
+  const int64_t Vd = 2 * V, DP = spr_padded_directions(D);
+  if (L.gcap < 1 || T >= ((int64_t)1 << 29) ||
+      D * B * T >= ((int64_t)1 << 29) || P * D >= 0x7fffffffll ||
+      Vd >= 0x7fffffffll)
+    return SPR_OK;
+  auto *row_ptr = reinterpret_cast<int64_t *>(ws + L.row_ptr);
+  auto *src = reinterpret_cast<int32_t *>(ws + L.src);
+  auto *lag = reinterpret_cast<int32_t *>(ws + L.lag);
+  auto *dir = reinterpret_cast<int32_t *>(ws + L.dir);
+  auto *coef = reinterpret_cast<double *>(ws + L.coef);
+  auto *hkey = reinterpret_cast<uint64_t *>(ws + L.coef);
+  auto *start = reinterpret_cast<int32_t *>(ws + L.start);
+  auto *rowid = reinterpret_cast<int32_t *>(ws + L.rowid);
+  auto *grp_ptr = reinterpret_cast<int64_t *>(ws + L.grp_ptr);
+  auto *grp_key = reinterpret_cast<int32_t *>(ws + L.grp_key);
+  auto *scal = reinterpret_cast<double *>(ws + L.scal);
+  auto *bad = reinterpret_cast<int32_t *>(ws + L.bad);
+  int rc;
+  if ((rc = fill<int32_t>(bad, 1, 0, s))) return rc;
+  hipLaunchKernelGGL(dense_row_hash_kernel, dim3((unsigned)P), dim3(256), 0,
+                     s, P, D, B, row_ptr, src, fft, hkey);
+  SPR_LAUNCH_CHECK();
+  int npad = 1;
+  while (npad < std::min<int64_t>(P, kSortMax)) npad <<= 1;  // rows <= P
+  const size_t lds =
+      (size_t)npad * (sizeof(uint64_t) + sizeof(int32_t)) + sizeof(int32_t);
+  SPR_HIP(hipFuncSetAttribute((const void *)dense_hash_sort_kernel,
+                              hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)lds));
+  hipLaunchKernelGGL(dense_hash_sort_kernel, dim3((unsigned)P), dim3(1024),
+                     lds, s, P, row_ptr, src, hkey, start, grp_ptr, bad);
+  SPR_LAUNCH_CHECK();
+  if ((rc = scan_exclusive(grp_ptr, P, s))) return rc;
+  int64_t h_groups = 0;
+  int32_t h_bad = 0;
+  SPR_HIP(hipMemcpyAsync(&h_groups, grp_ptr + P, sizeof(int64_t),
+                         hipMemcpyDeviceToHost, s));
+  SPR_HIP(hipMemcpyAsync(&h_bad, bad, sizeof(int32_t), hipMemcpyDeviceToHost,
+                         s));
+  SPR_HIP(hipStreamSynchronize(s));
+  if (h_bad || h_groups < 1 || h_groups > L.gcap) return SPR_OK;
+  if ((rc = spr_plan_from_dense(P, D, B, c, dt, Vd, row_ptr, src, dist, p2o,
+                                fft, lag, dir, coef, s)))
+    return rc;
+  hipLaunchKernelGGL(dense_group_fill_kernel, dim3((unsigned)ceil_div(P, 4)),
+                     dim3(256), 0, s, P, D, B, DP, Vd, row_ptr, grp_ptr, src,
+                     start, dir, coef, grp_key, rowid, lag, scal, bad);
+  SPR_LAUNCH_CHECK();
+  SPR_HIP(hipMemcpyAsync(&h_bad, bad, sizeof(int32_t), hipMemcpyDeviceToHost,
+                         s));
+  SPR_HIP(hipStreamSynchronize(s));
+  if (!h_bad) *groups = h_groups;
+  return SPR_OK;
+}
+
 // The body of spr_energy_exchange on a workspace `ws` of the dense layout
-// (sizes already checked by the caller).
-int energy_exchange_on(int64_t T, int64_t P, int64_t D, int64_t B,
-                       const double *e0dir, const double *d0,
+// (sizes already checked by the caller): the grouped two-pass exchange
+// when the F~ rows factorise, else the per-pair plan.
+int energy_exchange_on(const char *name, int64_t T, int64_t P, int64_t D,
+                       int64_t B, const double *e0dir, const double *d0,
                        const double *dist, const double *fft,
                        const int64_t *p2o, double c, double dt, int64_t K,
                        int64_t V, const int32_t *pairs, float *etc, char *ws,
@@ -1810,28 +2067,64 @@ int energy_exchange_on(int64_t T, int64_t P, int64_t D, int64_t B,
   auto *cur = reinterpret_cast<float *>(ws + L.cur);
   auto *lo = reinterpret_cast<float *>(ws + L.lo);
   auto *err = reinterpret_cast<int *>(ws + L.err);
+  last_dense_groups = 0;
   int rc;
   if ((rc = fill<int32_t>(err, 1, 0, s))) return rc;
   if ((rc = spr_exchange_init(P, D, B, T, e0dir, d0, c, dt, prev, etc, lo,
                               err, stream)))
     return rc;
   if (K >= 1 && V > 0) {
+    const int64_t Vd = 2 * V;
     if ((rc = fill<uint8_t>(vis, P * P, 0, s))) return rc;
     hipLaunchKernelGGL(pairs_to_vis_kernel, dim3((unsigned)ceil_div(V, 256)),
                        dim3(256), 0, s, P, V, pairs, vis);
     SPR_LAUNCH_CHECK();
     if ((rc = spr_plan_rows(P, vis, row_ptr, nullptr, stream))) return rc;
     if ((rc = spr_plan_rows(P, vis, row_ptr, src, stream))) return rc;
-    if ((rc = spr_plan_from_dense(P, D, B, c, dt, 2 * V, row_ptr, src, dist,
-                                  p2o, fft, lag, dir, coef, stream)))
+    int64_t G = 0;
+    if ((rc = dense_grouped_plan(T, P, D, B, dist, fft, p2o, c, dt, V, ws, L,
+                                 s, &G)))
       return rc;
-    for (int64_t k = 0; k < K; ++k) {
-      if ((rc = spr_exchange_step(P, D, B, T, row_ptr, src, lag, dir, coef,
-                                  prev, cur, etc, lo, stream)))
+    if (G > 0) {
+      // two passes per order; the total is updated every other order with
+      // two orders at once (spread mode 2), an odd last order on its own
+      auto *grp_ptr = reinterpret_cast<const int64_t *>(ws + L.grp_ptr);
+      auto *grp_key = reinterpret_cast<const int32_t *>(ws + L.grp_key);
+      auto *rowid = reinterpret_cast<const int32_t *>(ws + L.rowid);
+      auto *scal = reinterpret_cast<const double *>(ws + L.scal);
+      auto *gsum = reinterpret_cast<float *>(ws + L.gsum);
+      for (int64_t k = 0; k < K; ++k) {
+        if ((rc = launch_gather(P, B, T, Vd, row_ptr, grp_ptr, rowid, lag,
+                                scal, nullptr, prev, gsum, s)))
+          return rc;
+        const bool pair = k % 2 == 1;
+        if ((rc = spread_grouped(name, P, D, B, T, grp_ptr, grp_key, coef,
+                                 gsum, pair ? prev : nullptr, cur,
+                                 pair ? etc : nullptr, pair ? lo : nullptr,
+                                 stream)))
+          return rc;
+        float *t = prev;
+        prev = cur;
+        cur = t;
+      }
+      if (K % 2 == 1 &&
+          (rc = spr_exchange_total_add(P * D * B * T, prev, etc, lo, stream)))
         return rc;
-      float *t = prev;
-      prev = cur;
-      cur = t;
+      last_dense_groups = G;
+    } else {
+      // the sort may have reordered the rows: rebuild them in source order
+      if ((rc = spr_plan_rows(P, vis, row_ptr, src, stream))) return rc;
+      if ((rc = spr_plan_from_dense(P, D, B, c, dt, Vd, row_ptr, src, dist,
+                                    p2o, fft, lag, dir, coef, stream)))
+        return rc;
+      for (int64_t k = 0; k < K; ++k) {
+        if ((rc = spr_exchange_step(P, D, B, T, row_ptr, src, lag, dir, coef,
+                                    prev, cur, etc, lo, stream)))
+          return rc;
+        float *t = prev;
+        prev = cur;
+        cur = t;
+      }
     }
   }
   if ((rc = spr_exchange_finalize(P * D * B * T, etc, lo, etc, stream)))
@@ -1841,8 +2134,7 @@ int energy_exchange_on(int64_t T, int64_t P, int64_t D, int64_t B,
   SPR_HIP(hipStreamSynchronize(s));
   if (h_err)
     return fail(SPR_ERR_RANGE,
-                "spr_energy_exchange: a source delay is beyond the "
-                "histogram length");
+                "%s: a source delay is beyond the histogram length", name);
   return SPR_OK;
 }
 
@@ -1908,9 +2200,9 @@ extern "C" int spr_energy_exchange(int64_t T, int64_t P, int64_t D, int64_t B,
     return fail(SPR_ERR_WORKSPACE,
                 "spr_energy_exchange: workspace of %lld bytes, need %lld",
                 (long long)workspace_bytes, (long long)L.total);
-  return energy_exchange_on(T, P, D, B, e0dir, d0, dist, fft, p2o, c, dt, K,
-                            V, pairs, etc, static_cast<char *>(workspace), L,
-                            stream);
+  return energy_exchange_on("spr_energy_exchange", T, P, D, B, e0dir, d0,
+                            dist, fft, p2o, c, dt, K, V, pairs, etc,
+                            static_cast<char *>(workspace), L, stream);
 }
 
 extern "C" int spr_energy_exchange_noalloc(
@@ -1931,6 +2223,7 @@ extern "C" int spr_energy_exchange_noalloc(
                 (long long)L.total, (long long)SPR_NOALLOC_SCRATCH_BYTES);
   void *ws = nullptr;
   if ((rc = noalloc_scratch(&ws))) return rc;
-  return energy_exchange_on(T, P, D, B, e0dir, d0, dist, fft, p2o, c, dt, K,
-                            V, pairs, etc, static_cast<char *>(ws), L, stream);
+  return energy_exchange_on("spr_energy_exchange_noalloc", T, P, D, B, e0dir,
+                            d0, dist, fft, p2o, c, dt, K, V, pairs, etc,
+                            static_cast<char *>(ws), L, stream);
 }

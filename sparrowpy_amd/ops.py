@@ -265,6 +265,12 @@ def energy_exchange(n_samples, energy_0_directivity, distance_0,
     return etc
 
 
+def energy_exchange_groups():
+    """Groups of the grouped plan the last energy_exchange call on this
+    thread ran from the dense F~ (0: the per-pair plan ran)."""
+    return int(_hip.lib().spr_energy_exchange_groups())
+
+
 @dataclasses.dataclass
 class ExchangePlan:
     """Destination-major pair table of one scene (see sparrow_hip.h)."""
