@@ -34,7 +34,7 @@ def _scene(name, brdf="scattering"):
     if name == "canyon":
         return scenes.street_canyon(brdf=brdf)
     if name == "shoebox":
-        return scenes.directional_shoebox()
+        return scenes.directional_shoebox(brdf=brdf)
     if name == "small":
         return scenes.small_shoebox()
     raise ValueError(name)
@@ -420,6 +420,78 @@ def bench_inverse(args, log):
     }
 
 
+def bench_dense(args, log):
+    """The reference-shaped entry point spr_energy_exchange (dense F~
+    [P,P,D,B] in float64, RadiosityFast.py:1073-1145) on the directional
+    shoebox with per-wall directional-scattering BRDFs: time per order of
+    the grouped exchange it runs on factorising F~ rows, and of the
+    per-pair plan it falls back to, timed on the same F~ with every row
+    perturbed by up to 1e-3 (no longer factorising).  Per order = (time of
+    3 orders - time of 1 order) / 2, each call including its plan."""
+    import torch
+    from sparrowpy_amd import DirectionalRadiosityFast, ops
+    from sparrowpy_amd.coordinates import Coordinates, FrequencyData
+    sc = _scene("shoebox", "directional")
+    r = DirectionalRadiosityFast.from_polygon(sc.walls, sc.patch_size)
+    for idx, br in sc.brdf_groups:
+        r.set_wall_brdf(idx, br, sc.directions, sc.directions)
+    r.set_air_attenuation(FrequencyData(sc.air_attenuation, sc.frequencies))
+    r.bake_geometry()
+    r.init_source_energy(Coordinates(*sc.sources[0]))
+    b, g = r._bake, r._geometry()
+    P, D, B, T = r.n_patches, int(b["D"]), int(b["n_bins"]), sc.n_samples
+    c, dt = sc.speed_of_sound, sc.time_resolution
+    fft = ops.form_factors_with_directivity_dim(
+        b["vis"], b["ff"], b["n_bins"], g["centers"], g["areas"], b["air"],
+        g["wall_ids"], b["brdf"], b["brdf_index"], b["sources"],
+        b["receivers"])
+    ctr = r.patches_center
+    dd = ctr[:, None, :] - ctr[None, :, :]
+    dist = np.sqrt((dd[..., 0] * dd[..., 0] + dd[..., 1] * dd[..., 1])
+                   + dd[..., 2] * dd[..., 2])
+    dist = torch.tensor(dist, dtype=torch.float64, device="cuda")
+    e0d = r._dev["energy_init_source"]
+    d0 = r._dev["distance_patches_to_source"]
+    V = int(b["pairs"].shape[0])
+
+    def run(K):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        out = ops.energy_exchange(T, e0d, d0, dist, fft, b["p2o"], c, dt, K,
+                                  b["pairs"])
+        torch.cuda.synchronize()
+        del out
+        return time.perf_counter() - t0, ops.energy_exchange_groups()
+
+    res = {}
+    for kind in ("grouped", "per_pair"):
+        if kind == "per_pair":
+            gen = torch.Generator(device="cuda").manual_seed(0)
+            fft.mul_(1 + 1e-3 * torch.rand(fft.shape, generator=gen,
+                                           device="cuda", dtype=fft.dtype))
+        run(1)  # warm-up
+        t1, groups = run(1)
+        t3, _ = run(3)
+        per = (t3 - t1) / 2
+        res[kind] = {"ms_per_order": per * 1e3,
+                     "plan_ms": (t1 - per) * 1e3, "groups": groups}
+        log(f"dense entry, {kind}: {per * 1e3:.1f} ms per order, plan "
+            f"{(t1 - per) * 1e3:.1f} ms, {groups} groups")
+    return {
+        "metric": "dense-F~ exchange time per order (spr_energy_exchange)",
+        "value": res["grouped"]["ms_per_order"], "unit": "ms/order",
+        "n_gpus": 1, "higher_is_better": False, "vs_baseline": None,
+        "dtype": "f32 state, f64 accumulate",
+        "data": "synthetic (seeded directional shoebox)",
+        "config": {"workload": f"{sc.name}: P={P}, D={D}, B={B}, T={T}, "
+                               f"{2 * V} directed pairs, dense F~ "
+                               f"{P * P * D * B * 8 / 2**30:.1f} GiB"},
+        "grouped": res["grouped"], "per_pair": res["per_pair"],
+        "speedup": res["per_pair"]["ms_per_order"] /
+        res["grouped"]["ms_per_order"],
+    }
+
+
 def _tiling_str(t):
     rows, block, r = t
     if not rows:
@@ -611,10 +683,11 @@ def main(argv=None):
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default="canyon",
                     choices=["canyon", "shoebox", "small", "batched",
-                             "inverse", "adjoint"],
+                             "inverse", "adjoint", "dense"],
                     help="adjoint: only the adjoint/forward timings of the "
                          "default run (diffuse + specular and general "
-                         "BRDF rows)")
+                         "BRDF rows); dense: the reference-shaped "
+                         "spr_energy_exchange on a dense F~")
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="target CPU time of the oracle baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -690,10 +763,11 @@ def main(argv=None):
     from sparrowpy_amd import DirectionalRadiosityFast, ops
     from sparrowpy_amd.coordinates import Coordinates, FrequencyData
 
-    if args.config in ("batched", "inverse"):
+    if args.config in ("batched", "inverse", "dense"):
         def blog(msg):
             print(f"[bench] {msg}", file=sys.stderr, flush=True)
-        fn = bench_batched if args.config == "batched" else bench_inverse
+        fn = {"batched": bench_batched, "inverse": bench_inverse,
+              "dense": bench_dense}[args.config]
         print(json.dumps(fn(args, blog)), flush=True)
         return 0
     if args.config == "adjoint":
