@@ -11,11 +11,13 @@
 // i that see j, shifted by their delays, and writes the D rows
 // cur[j, :, b, :].
 //
-// Kernel mapping: a workgroup owns (j, b, a block of BS*R time bins); each
-// thread owns R bins (strided by BS so every load instruction is one
-// contiguous 256-byte wavefront access) and accumulates all DP directions
-// in float64 registers.  The per-pair coefficient row coef[e, b, :] is
-// uniform across the workgroup and is read through the scalar unit.
+// Kernel mapping of the per-pair plan: a workgroup owns (j, b, a block of
+// BS*R time bins); each thread owns R bins (strided by BS so every load
+// instruction is one contiguous 256-byte wavefront access) and accumulates
+// all DP directions in float64 registers.  The per-pair coefficient row
+// coef[e, b, :] is uniform across the workgroup and is staged in LDS.  The
+// grouped plans (group sums, then a spread by each group's BRDF row) are
+// below, with their construction from the geometry or from a dense F~.
 #include <mutex>
 
 #include "spr_common.hpp"
