@@ -497,6 +497,18 @@ int spr_energy_exchange(int64_t T, int64_t P, int64_t D, int64_t B,
                         void *workspace, int64_t workspace_bytes,
                         void *stream);
 
+/* hkey[e] (uint64, Vd entries) = hash of entry e's F~ row fft[src[e], j]
+ * (destination j of row_ptr's row holding e) with every band divided by its
+ * first element of largest magnitude and rounded to multiples of 2^-30:
+ * rows that are band by band multiples of one another hash alike (up to
+ * that rounding), the matching that spr_energy_exchange groups by.  A
+ * grouped plan built on it must still check every entry against its
+ * group's row.  Helper of the grouped plan from a saved dense F~; no
+ * counterpart in the reference. */
+int spr_plan_dense_hash(int64_t P, int64_t D, int64_t B,
+                        const int64_t *row_ptr, const int32_t *src,
+                        const double *fft, uint64_t *hkey, void *stream);
+
 /* Groups of the grouped plan the last spr_energy_exchange(_noalloc) call
  * on this thread ran, 0 if it ran the per-pair plan.
  * Diagnostic of spr_energy_exchange; no counterpart in the reference. */

@@ -151,6 +151,8 @@ SYMBOLS = {
     "spr_energy_exchange": (_int, [_i64, _i64, _i64, _i64, _vp, _vp, _vp,
                                    _vp, _vp, _dbl, _dbl, _i64, _i64, _vp,
                                    _vp, _vp, _i64, _vp]),
+    "spr_plan_dense_hash": (_int, [_i64, _i64, _i64, _vp, _vp, _vp, _vp,
+                                   _vp]),
     "spr_energy_exchange_groups": (_i64, []),
     "spr_energy_exchange_noalloc": (_int, [_i64, _i64, _i64, _i64, _vp, _vp,
                                            _vp, _vp, _vp, _dbl, _dbl, _i64,

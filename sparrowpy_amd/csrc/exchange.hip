@@ -1633,6 +1633,19 @@ extern "C" int spr_plan_grouped_fill(
   return SPR_OK;
 }
 
+extern "C" int spr_plan_dense_hash(int64_t P, int64_t D, int64_t B,
+                                   const int64_t *row_ptr, const int32_t *src,
+                                   const double *fft, uint64_t *hkey,
+                                   void *stream) {
+  SPR_REQUIRE(P > 0 && D > 0 && B > 0, "spr_plan_dense_hash: bad sizes");
+  SPR_REQUIRE(row_ptr && src && fft && hkey,
+              "spr_plan_dense_hash: null pointer");
+  hipLaunchKernelGGL(dense_row_hash_kernel, dim3((unsigned)P), dim3(256), 0,
+                     as_stream(stream), P, D, B, row_ptr, src, fft, hkey);
+  SPR_LAUNCH_CHECK();
+  return SPR_OK;
+}
+
 extern "C" int spr_brdf_diffuse_specular(int64_t NK, int64_t D, int64_t B,
                                          const double *brdf, double *a,
                                          double *c, int32_t *spec_dir,

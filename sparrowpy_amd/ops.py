@@ -449,6 +449,92 @@ class GroupedPlan:
                    rowid[:Vd], lag[:Vd], scal[:, :Vd].contiguous(), rtab)
 
 
+    # entry check of from_dense: |F~_e - s_e F~_key| <= ROW_TOL max|F~_e|
+    # per band, element by element
+    ROW_TOL = 2.0 ** -28
+
+    @classmethod
+    def from_dense(cls, vis, distance_ij, patch_2_out_directions,
+                   form_factors_tilde, speed_of_sound, time_resolution,
+                   max_group_fraction=0.25):
+        """Grouped plan of a dense F~ [P,P,D,B] whose rows factorise, or
+        None.
+
+        F~[i, j, :, b] = f_ij exp(-air_b d_ij) brdf[wall_i, s_ij, :, b]
+        (RadiosityFast.py:1234-1272), so the entries that share a BRDF row
+        have rows that are band by band multiples of one vector.  Entries
+        are keyed by the hash of their normalised row
+        (spr_plan_dense_hash); keys are global, so destination tiles can
+        merge entries of neighbouring destinations.  A key's table row is
+        the F~ row of its first entry, and each entry's per-band scalar is
+        s_eb = F~_e[piv] / F~_key[piv] at the key row's first element of
+        largest magnitude.  Every entry is checked:
+        |F~_e - s_e F~_key| <= 2^-28 max_d |F~_e| per band.  None when a
+        check fails (a hash collision, or values that are not numbers) or
+        when the groups exceed max_group_fraction of the entries, where the
+        per-pair plan (ExchangePlan.from_dense) is the faster one.
+        """
+        base = ExchangePlan.from_dense(vis, distance_ij,
+                                       patch_2_out_directions,
+                                       form_factors_tilde, speed_of_sound,
+                                       time_resolution)
+        P, D, B, Vd = base.P, base.D, base.B, base.Vd
+        if Vd == 0:
+            return None
+        fft = _f64(form_factors_tilde)
+        dev = fft.device
+        hkey = torch.empty(Vd, dtype=torch.int64, device=dev)
+        call("spr_plan_dense_hash", P, D, B, ptr(base.row_ptr),
+             ptr(base.src), ptr(fft), ptr(hkey), stream_ptr())
+        _, key = torch.unique(hkey, return_inverse=True)
+        NK = int(key.max().item()) + 1
+        first = torch.full((NK,), Vd, dtype=torch.int64, device=dev)
+        first.scatter_reduce_(0, key, torch.arange(Vd, device=dev), "amin")
+        coef = base.coef                           # [Vd, B, DP]
+        rtab = coef[first].contiguous()            # [NK, B, DP]
+        piv = rtab[..., :D].abs().argmax(dim=-1)   # first maximum
+        rpiv = rtab.gather(2, piv[..., None])[..., 0]
+        scal = torch.empty((Vd, B), dtype=torch.float64, device=dev)
+        step = max(1, (1 << 26) // (B * rtab.shape[2]))
+        for e0 in range(0, Vd, step):
+            k = key[e0:e0 + step]
+            f = coef[e0:e0 + step, :, :D]
+            rp = rpiv[k]
+            fp = f.gather(2, piv[k][..., None])[..., 0]
+            sc = torch.where(rp != 0, fp / torch.where(rp != 0, rp, 1.0),
+                             torch.zeros_like(fp))
+            err = (f - sc[..., None] * rtab[k, :, :D]).abs()
+            bound = cls.ROW_TOL * f.abs().amax(dim=-1, keepdim=True)
+            if not bool((err <= bound).all().item()):
+                return None
+            scal[e0:e0 + step] = sc
+        # each destination's entries by key (stable: sources ascending)
+        counts = base.row_ptr[1:] - base.row_ptr[:-1]
+        j = torch.repeat_interleave(torch.arange(P, device=dev), counts)
+        comp, perm = torch.sort(j * NK + key, stable=True)
+        new = torch.ones(Vd, dtype=torch.bool, device=dev)
+        new[1:] = comp[1:] != comp[:-1]
+        starts = torch.nonzero(new)[:, 0]
+        G = int(starts.numel())
+        if G > max_group_fraction * Vd:
+            return None
+        grp_ptr = torch.zeros(P + 1, dtype=torch.int64, device=dev)
+        grp_ptr[1:] = torch.cumsum(torch.bincount(comp[starts] // NK,
+                                                  minlength=P), 0)
+        grp_start = torch.empty(G + 1, dtype=torch.int64, device=dev)
+        grp_start[:G] = starts
+        grp_start[G] = Vd
+        last = torch.ones(Vd, dtype=torch.bool, device=dev)
+        last[:-1] = new[1:]
+        src = base.src[perm].to(torch.int64)
+        rowid = (src * D + base.dir[perm].to(torch.int64)).to(torch.int32)
+        lag = base.lag[perm] | (last.to(torch.int32) * cls.LAST_IN_GROUP)
+        grp_key = key[perm][starts].to(torch.int32)
+        return cls(P, D, B, base.row_ptr, grp_ptr, grp_start, grp_key,
+                   rowid.contiguous(), lag.contiguous(),
+                   scal[perm].t().contiguous(), rtab)
+
+
 @dataclasses.dataclass
 class TilePlan:
     """Destination tiles of a GroupedPlan: JT rows merged into M distinct

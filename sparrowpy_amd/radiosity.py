@@ -687,7 +687,9 @@ class DirectionalRadiosityFast:
         ``kind``: "grouped", "ds" (diffuse + specular BRDF rows), "generic"
         or None (automatic: "grouped" when the sources of a destination
         share few BRDF rows, else "ds" when every BRDF row is a constant
-        plus one specular entry, else "generic").
+        plus one specular entry, else "generic").  On a saved dense F~
+        (restored state) the plan is "grouped" when its rows factorise
+        into few groups and "generic" otherwise.
         """
         key = (float(speed_of_sound), float(time_resolution), kind,
                self.TILE_ROWS, self.TILE_MIN_MERGE, self.TILE_BLOCK,
@@ -699,14 +701,25 @@ class DirectionalRadiosityFast:
         plan = None
         if b.get("fft_dense") is not None:
             # a saved F~ is used as given (reference: the attribute read
-            # by _energy_exchange)
+            # by _energy_exchange): grouped when its rows factorise into
+            # few groups, else per pair
             ctr = self.patches_center
             d = ctr[:, None, :] - ctr[None, :, :]
             dist = np.sqrt((d[..., 0] * d[..., 0] + d[..., 1] * d[..., 1])
                            + d[..., 2] * d[..., 2])
-            plan = ops.ExchangePlan.from_dense(
-                b["vis"], dist, b["p2o"], b["fft_dense"], speed_of_sound,
-                time_resolution)
+            if kind in (None, "grouped"):
+                plan = ops.GroupedPlan.from_dense(
+                    b["vis"], dist, b["p2o"], b["fft_dense"], speed_of_sound,
+                    time_resolution, self.GROUPED_MAX_GROUP_FRACTION)
+                if plan is None and kind == "grouped":
+                    raise ValueError("the saved form_factors_tilde rows do "
+                                     "not factorise into few groups")
+                if plan is not None and self.TILE_ROWS:
+                    self._add_tiles(plan)
+            if plan is None:
+                plan = ops.ExchangePlan.from_dense(
+                    b["vis"], dist, b["p2o"], b["fft_dense"], speed_of_sound,
+                    time_resolution)
             self._plan = (key, plan)
             return plan
         args = (b["vis"], g["centers"], g["areas"], b["ff"], b["p2o"],
@@ -723,11 +736,7 @@ class DirectionalRadiosityFast:
                     plan.n_groups > self.GROUPED_MAX_GROUP_FRACTION * plan.Vd):
                 plan = None
             if plan is not None and self.TILE_ROWS:
-                plan.tiles = ops.tile_plan(plan, self.TILE_ROWS,
-                                           self.TILE_MIN_MERGE,
-                                           self._tile_order())
-                if plan.tiles is not None:
-                    plan.tiles.R = self.TILE_R
+                self._add_tiles(plan)
         if plan is None and kind in (None, "ds"):
             try:
                 plan = ops.diffuse_specular_plan(*args)
@@ -745,6 +754,14 @@ class DirectionalRadiosityFast:
             plan = ops.ExchangePlan.from_geometry(*args)
         self._plan = (key, plan)
         return plan
+
+    def _add_tiles(self, plan):
+        """Destination tiles of TILE_ROWS rows for a grouped plan (kept only
+        when they merge at least TILE_MIN_MERGE entries into one)."""
+        plan.tiles = ops.tile_plan(plan, self.TILE_ROWS, self.TILE_MIN_MERGE,
+                                   self._tile_order())
+        if plan.tiles is not None:
+            plan.tiles.R = self.TILE_R
 
     def calculate_energy_exchange(self, speed_of_sound, etc_time_resolution,
                                   etc_duration, max_reflection_order=-1,

@@ -145,3 +145,88 @@ def test_dense_entry_constant_rows_and_zero_band(ops):
           f"max rel err {err:.3e}")
     assert groups == want
     assert err <= EXCHANGE_RTOL
+
+
+def _dist(ctr):
+    """Pair distances as the class computes them for a saved F~."""
+    d = ctr[:, None, :] - ctr[None, :, :]
+    return np.sqrt((d[..., 0] * d[..., 0] + d[..., 1] * d[..., 1])
+                   + d[..., 2] * d[..., 2])
+
+
+def _class_scene():
+    """The shoebox of the tests above as a DirectionalRadiosityFast with the
+    same per-wall BRDFs (same draws), baked, with the source set."""
+    from sparrowpy_amd import DirectionalRadiosityFast
+    from sparrowpy_amd.coordinates import Coordinates
+    r = DirectionalRadiosityFast.from_polygon(
+        testing.shoebox_room_stub(3, 2.5, 2), .5)
+    rng = np.random.default_rng(0)
+    c = samplings.hemisphere(samplings.sph_gaussian(sh_order=3))
+    freqs = np.array([250., 500.])
+    for w in range(6):
+        br = B.create_from_scattering(
+            c, c, FrequencyData(rng.uniform(.1, .9, (1, 2)), freqs),
+            FrequencyData(rng.uniform(.05, .3, (1, 2)), freqs))
+        r.set_wall_brdf([w], br, c, c)
+    r.set_air_attenuation(FrequencyData(rng.uniform(1e-4, 5e-3, 2), freqs))
+    r.bake_geometry()
+    r.init_source_energy(Coordinates(*SOURCE))
+    return r
+
+
+def test_saved_dense_fft_runs_grouped_plan(ops):
+    """A state restored with its dense F~ plans the exchange on it: the
+    grouped plan with keys shared across destinations (so destination
+    tiles merge entries), the same groups per destination as
+    spr_energy_exchange finds on that F~, and the histograms of the
+    original object (whose plan comes from the geometry) within 1e-6."""
+    from sparrowpy_amd import DirectionalRadiosityFast
+    from sparrowpy_amd.coordinates import Coordinates
+    r = _class_scene()
+    state = r.to_dict()
+    assert state["form_factors_tilde"] is not None
+    rr = DirectionalRadiosityFast.from_dict(state)
+    rr.init_source_energy(Coordinates(*SOURCE))
+    plan = rr._exchange_plan(C, DT)
+    assert isinstance(plan, ops.GroupedPlan) and plan.tiles is not None
+    fft = np.asarray(state["form_factors_tilde"])
+    ops.energy_exchange(50, rr._energy_init_source,
+                        rr._distance_patches_to_source,
+                        _dist(rr.patches_center),
+                        fft, rr._patch_2_brdf_outgoing_index, C, DT, 1,
+                        rr._visible_patches)
+    assert plan.n_groups == ops.energy_exchange_groups() > 0
+    T, K = 400, 6
+    r.calculate_energy_exchange(C, DT, T * DT, K)
+    rr.calculate_energy_exchange(C, DT, T * DT, K)
+    err = rel_err_within_db(rr._energy_exchange_etc, r._energy_exchange_etc)
+    print(f"[saved dense F~] {plan.n_groups} groups, {plan.tiles.M} merged "
+          f"tile entries for {plan.Vd} entries; max rel err {err:.2e}")
+    assert err <= 1e-6
+
+
+def test_saved_dense_fft_per_pair_rows(ops):
+    """Rows that do not factorise: the per-pair plan (and kind="grouped"
+    refuses), with the oracle's histograms on that F~."""
+    from sparrowpy_amd import DirectionalRadiosityFast
+    from sparrowpy_amd.coordinates import Coordinates
+    r = _class_scene()
+    state = r.to_dict()
+    rng = np.random.default_rng(5)
+    fft = np.asarray(state["form_factors_tilde"])
+    state["form_factors_tilde"] = fft * (1 + .05 * rng.random(fft.shape))
+    rr = DirectionalRadiosityFast.from_dict(state)
+    rr.init_source_energy(Coordinates(*SOURCE))
+    with pytest.raises(ValueError, match="factorise"):
+        rr._exchange_plan(C, DT, "grouped")
+    assert isinstance(rr._exchange_plan(C, DT), ops.ExchangePlan)
+    T, K = 300, 4
+    rr.calculate_energy_exchange(C, DT, T * DT, K)
+    ref = orc.exchange(T, rr._energy_init_source,
+                       rr._distance_patches_to_source,
+                       _dist(rr.patches_center),
+                       state["form_factors_tilde"],
+                       rr._patch_2_brdf_outgoing_index, C, DT, K,
+                       rr._visible_patches, nthreads=8)
+    assert rel_err_within_db(rr._energy_exchange_etc, ref) <= EXCHANGE_RTOL

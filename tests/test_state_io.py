@@ -133,8 +133,8 @@ def test_io_within_simulation(gpu, tmp_path):
     rr = _roundtrip(r, tmp_path / "s.npz")
     r.calculate_energy_exchange(343, 1 / 1000, 1, 3, recalculate=True)
     rr.calculate_energy_exchange(343, 1 / 1000, 1, 3, recalculate=True)
-    # the restored object runs the exchange on the saved dense F~, the
-    # original on its pair table: same terms, same order per bin
+    # the restored object runs the exchange on the saved dense F~ (grouped
+    # when its rows factorise), the original on the plan from its geometry
     a, b = rr.to_dict(), r.to_dict()
     etc_a, etc_b = a.pop("energy_exchange_etc"), b.pop("energy_exchange_etc")
     assert DirectionalRadiosityFast.from_dict(a) == \
