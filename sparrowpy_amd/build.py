@@ -25,6 +25,9 @@ HIPFLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-fPIC", "-std=c++17",
     "-ffp-contract=off", "-Wall", "-Wno-unused-function",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # matrix-core accumulators in VGPRs: with AGPRs the compiler copied the
+    # per-pair kernel's 64 accumulator registers out and back every step
+    "-mllvm", "-amdgpu-mfma-vgpr-form",
 ]
 
 

@@ -12,12 +12,13 @@
 // cur[j, :, b, :].
 //
 // Kernel mapping of the per-pair plan: a workgroup owns (j, b, a block of
-// BS*R time bins); each thread owns R bins (strided by BS so every load
-// instruction is one contiguous 256-byte wavefront access) and accumulates
-// all DP directions in float64 registers.  The per-pair coefficient row
-// coef[e, b, :] is uniform across the workgroup and is staged in LDS.  The
-// grouped plans (group sums, then a spread by each group's BRDF row) are
-// below, with their construction from the geometry or from a dense F~.
+// time bins).  With 16 or more (padded) directions the order is a matrix
+// product on the FP64 matrix cores (exchange_mfma_kernel); below that each
+// thread owns R bins (strided by BS so every load instruction is one
+// contiguous 256-byte wavefront access) and accumulates all DP directions
+// in float64 registers, the coefficient rows coef[e, b, :] staged in LDS.
+// The grouped plans (group sums, then a spread by each group's BRDF row)
+// are below, with their construction from the geometry or from a dense F~.
 #include <mutex>
 
 #include "spr_common.hpp"
@@ -334,6 +335,137 @@ __global__ __launch_bounds__(BS) void exchange_step_kernel(
     }
   }
   store_tile<DP, R, BS>(acc, D, B, T, j, b, t0, cur, tot_hi, tot_lo);
+}
+
+// Per-pair plan on the matrix cores (DP >= 16).  For destination j and band
+// b one order is the product
+//   cur[j, :, b, t] = sum_e coef[e, b, :] x_e[t],
+//   x_e[t] = prev[src_e * D + dir_e, b, t - lag_e]  (0 for t < lag_e),
+// of the [DP x E_j] coefficient rows and the [E_j x T] shifted source
+// histograms, formed with v_mfma_f64_16x16x4f64: K = 4 entries per step;
+// lane l holds A[l & 15][l >> 4] (coefficient of direction l & 15 of entry
+// l >> 4), B[l >> 4][l & 15] (bin l & 15 of that entry) and, in register
+// r, D[(l >> 4) + 4r][l & 15].  A wave owns MT = DP / 16 direction tiles
+// by NT tiles of 16 bins (MT * NT = 8 accumulators of 4 doubles); each
+// step loads, per lane, its entry's MT coefficients and NT source bins,
+// the next step's while the current one is multiplied.  Entries past the
+// row end are zero.  FP64 throughout; the sums' order differs from the
+// vector kernel's (float64 rounding).
+typedef double mfma_d4 __attribute__((ext_vector_type(4)));
+
+template <int DP, int NT>
+__global__ __launch_bounds__(256) void exchange_mfma_kernel(
+    int D, int B, int T, int n_tblk, int64_t nblocks,
+    const int64_t *__restrict__ row_ptr, const int32_t *__restrict__ src,
+    const int32_t *__restrict__ lag, const int32_t *__restrict__ dir,
+    const double *__restrict__ coef, const float *__restrict__ prev,
+    float *__restrict__ cur, float *__restrict__ tot_hi,
+    float *__restrict__ tot_lo) {
+  constexpr int MT = DP / 16;
+  constexpr int WBINS = 16 * NT;  // bins per wave
+  static_assert(MT * NT == 8, "eight accumulator tiles per wave");
+  const int64_t tile = xcd_tile(blockIdx.x, nblocks);
+  const int tb = (int)(tile % n_tblk);
+  const int64_t jb = tile / n_tblk;
+  const int b = (int)(jb % B);
+  const int64_t j = jb / B;
+  const int lane = (int)threadIdx.x & 63;
+  const int kk = lane >> 4, cc = lane & 15;
+  const int tw = (tb * 4 + ((int)threadIdx.x >> 6)) * WBINS;
+  const int64_t BT = (int64_t)B * T;
+  const int64_t e_begin = row_ptr[j], e_end = row_ptr[j + 1];
+
+  mfma_d4 acc[MT][NT];
+#pragma unroll
+  for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt) acc[mt][nt] = mfma_d4{0.0, 0.0, 0.0, 0.0};
+
+  // Software pipeline, all loads branch-free: the metadata of step s + 2
+  // and the operands of step s + 1 are loaded while step s is multiplied,
+  // and a load's value is first used one step after it was issued.  An
+  // entry past the row end reads the row's last entry, a bin outside the
+  // source row (t < lag, or t - lag >= T) reads the row's first bin, and
+  // selects at the use zero both.  (Each 16-lane group reads its own source
+  // row, so a buffer descriptor would not be wave-uniform.)
+  struct Meta {
+    int64_t e;  // entry, clamped to the row
+    bool in;
+    int32_t src, dir, lag;
+  };
+  struct Ops {
+    double a[MT];
+    float x[NT];
+    bool in, ok[NT];
+  };
+  auto meta = [&](int64_t e0) {
+    Meta m;
+    const int64_t e = e0 + kk;
+    m.in = e < e_end;
+    m.e = m.in ? e : e_end - 1;
+    m.src = src[m.e];
+    m.dir = dir[m.e];
+    m.lag = lag[m.e];
+    return m;
+  };
+  auto issue = [&](const Meta &m) {
+    Ops o;
+    o.in = m.in;
+    const double *cf = coef + (m.e * B + b) * DP + cc;
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt) o.a[mt] = cf[mt * 16];
+    const float *row =
+        prev + ((int64_t)m.src * D + m.dir) * BT + (int64_t)b * T;
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt) {
+      const int ts = tw + nt * 16 + cc - m.lag;
+      o.ok[nt] = m.in && (unsigned)ts < (unsigned)T;
+      o.x[nt] = row[o.ok[nt] ? ts : 0];
+    }
+    return o;
+  };
+
+  if (e_end > e_begin) {  // uniform
+    Meta m1 = meta(e_begin + 4);
+    Ops cur_ops = issue(meta(e_begin));
+    for (int64_t e0 = e_begin; e0 < e_end; e0 += 4) {
+      const Meta m2 = meta(e0 + 8);
+      const Ops nxt = issue(m1);
+      double a[MT], x[NT];
+#pragma unroll
+      for (int mt = 0; mt < MT; ++mt) a[mt] = cur_ops.in ? cur_ops.a[mt] : 0.0;
+#pragma unroll
+      for (int nt = 0; nt < NT; ++nt)
+        x[nt] = cur_ops.ok[nt] ? (double)cur_ops.x[nt] : 0.0;
+#pragma unroll
+      for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt)
+          acc[mt][nt] = __builtin_amdgcn_mfma_f64_16x16x4f64(
+              a[mt], x[nt], acc[mt][nt], 0, 0, 0);
+      cur_ops = nxt;
+      m1 = m2;
+    }
+  }
+
+  // cur and the compensated total (as store_tile)
+#pragma unroll
+  for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int d = mt * 16 + kk + 4 * r;
+        const int t = tw + nt * 16 + cc;
+        if (d >= D || t >= T) continue;
+        const int64_t idx = (j * D + d) * BT + (int64_t)b * T + t;
+        const double v = acc[mt][nt][r];
+        cur[idx] = (float)v;
+        const double sum = ((double)tot_hi[idx] + (double)tot_lo[idx]) + v;
+        const float h = (float)sum;
+        tot_hi[idx] = h;
+        tot_lo[idx] = (float)(sum - (double)h);
+      }
 }
 
 // Grouped plan, pass 1 of an order: per group g of destination j's sources
@@ -819,6 +951,18 @@ int launch_step(int64_t P, int64_t D, int64_t B, int64_t T,
                 const int32_t *lag, const int32_t *dir, const double *coef,
                 const float *prev, float *cur, float *tot_hi, float *tot_lo,
                 hipStream_t s) {
+  if constexpr (DP >= 16) {
+    constexpr int NT = 8 / (DP / 16);
+    const int64_t n_tblk = ceil_div(T, (int64_t)4 * 16 * NT);
+    const int64_t nblocks = P * B * n_tblk;
+    SPR_REQUIRE(nblocks < (int64_t)0x7fffffff, "exchange grid too large");
+    hipLaunchKernelGGL((exchange_mfma_kernel<DP, NT>),
+                       dim3((unsigned)nblocks), dim3(256), 0, s, (int)D,
+                       (int)B, (int)T, (int)n_tblk, nblocks, row_ptr, src,
+                       lag, dir, coef, prev, cur, tot_hi, tot_lo);
+    SPR_LAUNCH_CHECK();
+    return SPR_OK;
+  }
   constexpr int BS = 256;
   const int64_t n_tblk = ceil_div(T, (int64_t)BS * R);
   const int64_t nblocks = P * B * n_tblk;
