@@ -425,26 +425,33 @@ __global__ __launch_bounds__(256) void exchange_mfma_kernel(
     return o;
   };
 
-  if (e_end > e_begin) {  // uniform
-    Meta m1 = meta(e_begin + 4);
-    Ops cur_ops = issue(meta(e_begin));
-    for (int64_t e0 = e_begin; e0 < e_end; e0 += 4) {
-      const Meta m2 = meta(e0 + 8);
-      const Ops nxt = issue(m1);
-      double a[MT], x[NT];
+  auto mult = [&](const Ops &o) {
+    double a[MT], x[NT];
 #pragma unroll
-      for (int mt = 0; mt < MT; ++mt) a[mt] = cur_ops.in ? cur_ops.a[mt] : 0.0;
+    for (int mt = 0; mt < MT; ++mt) a[mt] = o.in ? o.a[mt] : 0.0;
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt) x[nt] = o.ok[nt] ? (double)o.x[nt] : 0.0;
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt)
 #pragma unroll
       for (int nt = 0; nt < NT; ++nt)
-        x[nt] = cur_ops.ok[nt] ? (double)cur_ops.x[nt] : 0.0;
-#pragma unroll
-      for (int mt = 0; mt < MT; ++mt)
-#pragma unroll
-        for (int nt = 0; nt < NT; ++nt)
-          acc[mt][nt] = __builtin_amdgcn_mfma_f64_16x16x4f64(
-              a[mt], x[nt], acc[mt][nt], 0, 0, 0);
-      cur_ops = nxt;
-      m1 = m2;
+        acc[mt][nt] = __builtin_amdgcn_mfma_f64_16x16x4f64(
+            a[mt], x[nt], acc[mt][nt], 0, 0, 0);
+  };
+  if (e_end > e_begin) {  // uniform
+    // two operand sets in turn; the scheduling barriers keep each step's
+    // loads ahead of the previous step's products
+    Meta m1 = meta(e_begin + 4);
+    Ops oa = issue(meta(e_begin));
+    for (int64_t e0 = e_begin; e0 < e_end; e0 += 8) {
+      const Meta m2 = meta(e0 + 8);
+      const Ops ob = issue(m1);
+      __builtin_amdgcn_sched_barrier(0);
+      mult(oa);
+      m1 = meta(e0 + 12);
+      oa = issue(m2);
+      __builtin_amdgcn_sched_barrier(0);
+      mult(ob);
     }
   }
 

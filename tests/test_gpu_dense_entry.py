@@ -230,3 +230,25 @@ def test_saved_dense_fft_per_pair_rows(ops):
                        rr._patch_2_brdf_outgoing_index, C, DT, K,
                        rr._visible_patches, nthreads=8)
     assert rel_err_within_db(rr._energy_exchange_etc, ref) <= EXCHANGE_RTOL
+
+
+def test_dense_entry_without_brdf(ops):
+    """No BRDF (D = 1, the rows are scalars): one group per destination
+    that sees another patch; two bands with air attenuation."""
+    r = make_oracle(testing.shoebox_room_stub(3, 2.5, 2), .5)
+    freqs = np.array([250., 500.])
+    r.set_air_attenuation(np.array([1e-3, 4e-3]), freqs)
+    r.bake_geometry()
+    r.init_source_energy(np.asarray(SOURCE, float))
+    assert r.fft.shape[2] == 1
+    want = len({j for _, j, _ in _entries_without_brdf(r)})
+    out, groups, err = _run(ops, r, r.fft, 300, 5)
+    print(f"[dense no BRDF] groups {groups} (destinations {want}); "
+          f"max rel err {err:.3e}")
+    assert groups == want
+    assert err <= EXCHANGE_RTOL
+
+
+def _entries_without_brdf(r):
+    return [(int(i), int(j), 0) for a, b in np.asarray(r.visible_patches)
+            for i, j in ((a, b), (b, a))]
