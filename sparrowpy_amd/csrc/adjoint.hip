@@ -337,14 +337,19 @@ __global__ __launch_bounds__(256) void grouped_spread_adjoint_kernel(
 
 // gg[g,b,d] += sum_t mu[j_g,d,b,t] gsum[g,b,t] for the groups g of
 // destination j: per (j, band) the product of the group sums [G_j x T]
-// with mu[j,:,b,:]^T [T x D], in tiles of 16 NQ groups x 16 NR directions
-// (NQ from the groups left, NR from D: no tile rows beyond them; with
-// 64 x 64 tiles throughout the directional shoebox's launch took 15.3 ms),
-// the bins in chunks of 32 staged in LDS (float32: staged as float64 the
-// launch took 16.2 ms against 15.2).  Each lane sums NQ x NR (group,
-// direction) pairs in float64 and owns them: no atomics, and the result
-// does not depend on the launch's timing.
+// with mu[j,:,b,:]^T [T x D], on the FP64 matrix cores
+// (v_mfma_f64_16x16x4f64: lane l holds A[l & 15][l >> 4], B[l >> 4][l & 15]
+// and D[(l >> 4) + 4r][l & 15]).  Tiles of 16 NQ groups x 16 NR directions
+// (NQ from the groups left, NR from D); the bins in chunks of 32 staged in
+// LDS as float32, both operands read from there; wave q takes group tile
+// q against every direction tile (NR accumulators of 16 x 16).  Each
+// (group, direction) is summed by one lane in float64 and owned by it: no
+// atomics, and the result does not depend on the launch's timing.  The
+// directional shoebox's launch takes 5.5 ms (12.0 ms on the vector unit,
+// each lane summing NQ x NR pairs; 10.7 ms here without the prefetch of
+// the next chunk; 17.8 ms with chunks of 64 bins).
 constexpr int kTgChunk = 32;
+typedef double mfma_d4 __attribute__((ext_vector_type(4)));
 
 template <int NQ, int NR>
 __device__ __forceinline__ void table_grad_tile(
@@ -354,50 +359,57 @@ __device__ __forceinline__ void table_grad_tile(
     float (&sm)[64][kTgChunk + 1]) {
   constexpr int TC = kTgChunk;
   const int tid = (int)threadIdx.x;
-  const int tg = tid >> 4, td = tid & 15;
+  const int wave = tid >> 6, lane = tid & 63;
+  const int kk = lane >> 4, cc = lane & 15;
   const int64_t dstride = (int64_t)B * T;
-  double acc[NQ][NR];
+  mfma_d4 acc[NR];
 #pragma unroll
-  for (int q = 0; q < NQ; ++q)
-#pragma unroll
-    for (int r = 0; r < NR; ++r) acc[q][r] = 0.0;
-  for (int t0 = 0; t0 < T; t0 += TC) {
-    // 256 lanes stage 8 rows of 32 bins per step
-    const int col = tid % TC, t = t0 + col;
+  for (int r = 0; r < NR; ++r) acc[r] = mfma_d4{0.0, 0.0, 0.0, 0.0};
+  // 256 lanes stage 8 rows of 32 bins per step; the next chunk's values
+  // are loaded into registers while the current chunk is multiplied
+  const int col = tid % TC;
+  float pg[2 * NQ], pm[2 * NR];
+  auto fetch = [&](int t0) {
+    const int t = t0 + col;
 #pragma unroll
     for (int s = 0; s < 2 * NQ; ++s) {
-      const int row = tid / TC + 8 * s;
-      const int64_t g = gt + row;
-      sg[row][col] =
-          (g < g1 && t < T) ? gsum[(g * B + b) * (int64_t)T + t] : 0.f;
+      const int64_t g = gt + tid / TC + 8 * s;
+      pg[s] = (g < g1 && t < T) ? gsum[(g * B + b) * (int64_t)T + t] : 0.f;
     }
 #pragma unroll
     for (int s = 0; s < 2 * NR; ++s) {
       const int d = tid / TC + 8 * s;
-      sm[d][col] = (d < D && t < T) ? mrow[d * dstride + t] : 0.f;
+      pm[s] = (d < D && t < T) ? mrow[d * dstride + t] : 0.f;
     }
+  };
+  fetch(0);
+  for (int t0 = 0; t0 < T; t0 += TC) {
+#pragma unroll
+    for (int s = 0; s < 2 * NQ; ++s) sg[tid / TC + 8 * s][col] = pg[s];
+#pragma unroll
+    for (int s = 0; s < 2 * NR; ++s) sm[tid / TC + 8 * s][col] = pm[s];
     __syncthreads();
-#pragma unroll 2
-    for (int c = 0; c < TC; ++c) {
-      double a[NQ], m[NR];
+    if (t0 + TC < T) fetch(t0 + TC);  // uniform
+    if (wave < NQ) {  // uniform per wave
+#pragma unroll 4
+      for (int c = 0; c < TC; c += 4) {
+        const double a = (double)sg[16 * wave + cc][c + kk];
 #pragma unroll
-      for (int q = 0; q < NQ; ++q) a[q] = (double)sg[tg + 16 * q][c];
-#pragma unroll
-      for (int r = 0; r < NR; ++r) m[r] = (double)sm[td + 16 * r][c];
-#pragma unroll
-      for (int q = 0; q < NQ; ++q)
-#pragma unroll
-        for (int r = 0; r < NR; ++r) acc[q][r] = fma(a[q], m[r], acc[q][r]);
+        for (int r = 0; r < NR; ++r)
+          acc[r] = __builtin_amdgcn_mfma_f64_16x16x4f64(
+              a, (double)sm[16 * r + cc][c + kk], acc[r], 0, 0, 0);
+      }
     }
     __syncthreads();
   }
+  if (wave >= NQ) return;
 #pragma unroll
-  for (int q = 0; q < NQ; ++q)
+  for (int r = 0; r < NR; ++r)
 #pragma unroll
-    for (int r = 0; r < NR; ++r) {
-      const int64_t g = gt + tg + 16 * q;
-      const int d = td + 16 * r;
-      if (g < g1 && d < D) gg[(g * B + b) * DP + d] += acc[q][r];
+    for (int i = 0; i < 4; ++i) {
+      const int64_t g = gt + 16 * wave + kk + 4 * i;
+      const int d = 16 * r + cc;
+      if (g < g1 && d < D) gg[(g * B + b) * DP + d] += acc[r][i];
     }
 }
 
