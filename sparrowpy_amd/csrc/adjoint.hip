@@ -290,10 +290,12 @@ __global__ void injection_grad_kernel(
 //   dL/dE_k[r,b,t] += sum_{e: row_e = r} scal[b,e] nu[g_e,b,t+n_e] (gather^T)
 //   dL/dtable[key_g,d,b] += sum_t mu[j_g,d,b,t] gsum[g,b,t].
 
-// nu for every group of destination j: a workgroup per (j, band, 256
-// bins), one bin per lane; the lane keeps mu[j,:,b,t] as DPT doubles
-// (converted once), and the workgroup stages its groups' BRDF rows in LDS,
-// 16 at a time, read by broadcast.
+// nu for every group of destination j with up to 8 (padded) directions: a
+// workgroup per (j, band, 256 bins), one bin per lane; the lane keeps
+// mu[j,:,b,t] as DPT doubles (converted once), and the workgroup stages its
+// groups' BRDF rows in LDS, 16 at a time, read by broadcast.  (With 16 or
+// more directions grouped_spread_adjoint_mfma_kernel forms nu; this kernel
+// took 12.2 ms per call on the directional shoebox.)
 template <int DPT>
 __global__ __launch_bounds__(256) void grouped_spread_adjoint_kernel(
     int D, int B, int T, int DP, int n_tblk,
@@ -434,6 +436,81 @@ __global__ __launch_bounds__(256) void grouped_table_grad_kernel(
       table_grad_tile<2, NR>(D, B, T, DP, b, gt, g1, mrow, gsum, gg, sg, sm);
     else
       table_grad_tile<1, NR>(D, B, T, DP, b, gt, g1, mrow, gsum, gg, sg, sm);
+  }
+}
+
+// nu on the FP64 matrix cores (16, 32 or 64 padded directions): per
+// destination j and band b the product of the groups' BRDF rows
+// [G_j x DP] with mu[j,:,b,:] [DP x T], K = DP in steps of 4.  A workgroup
+// per (j, band, 4 x 16 NT bins), each wave 16 NT bins (NT = 2 at 64
+// directions, so that the operands fit four waves per SIMD, else 4): the
+// wave's mu operands (lane
+// l: direction 4s + (l >> 4), bin l & 15 of each 16-bin tile) are loaded
+// once into registers, then every tile of 16 groups is multiplied against
+// them, its BRDF rows staged in LDS.  Lane layout of
+// v_mfma_f64_16x16x4f64 as in grouped_table_grad_kernel.
+template <int DP>
+struct SpreadAdjMfma {
+  static constexpr int NT = DP == 64 ? 2 : 4;  // 16-bin tiles per wave
+  static constexpr int BINS = 4 * 16 * NT;      // bins per workgroup
+};
+
+template <int DP>
+__global__ __launch_bounds__(256) void grouped_spread_adjoint_mfma_kernel(
+    int D, int B, int T, int n_tblk, const int64_t *__restrict__ grp_ptr,
+    const int32_t *__restrict__ grp_key, const double *__restrict__ rtab,
+    const float *__restrict__ mu, float *__restrict__ nu) {
+  constexpr int KS = DP / 4, NT = SpreadAdjMfma<DP>::NT;
+  __shared__ double rts[16][DP + 1];
+  const int64_t blk = blockIdx.x;
+  const int tb = (int)(blk % n_tblk);
+  const int b = (int)((blk / n_tblk) % B);
+  const int64_t j = blk / ((int64_t)n_tblk * B);
+  const int64_t g0 = grp_ptr[j], g1 = grp_ptr[j + 1];
+  if (g0 == g1) return;  // uniform
+  const int tid = (int)threadIdx.x;
+  const int lane = tid & 63, kk = lane >> 4, cc = lane & 15;
+  const int tw = (tb * 4 + (tid >> 6)) * 16 * NT;
+  const int64_t dstride = (int64_t)B * T;
+  const float *m = mu + (j * D * B + b) * (int64_t)T;  // (j, 0, b, 0)
+  float mv[KS][NT];
+#pragma unroll
+  for (int s = 0; s < KS; ++s)
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt) {
+      const int d = 4 * s + kk, t = tw + 16 * nt + cc;
+      const bool ok = d < D && t < T;
+      const float v = m[ok ? d * dstride + t : 0];
+      mv[s][nt] = ok ? v : 0.f;
+    }
+  for (int64_t gt = g0; gt < g1; gt += 16) {
+    for (int idx = tid; idx < 16 * DP; idx += 256) {
+      const int gi = idx / DP, d = idx % DP;
+      rts[gi][d] = (gt + gi < g1 && d < D)
+                       ? rtab[((int64_t)grp_key[gt + gi] * B + b) * DP + d]
+                       : 0.0;
+    }
+    __syncthreads();
+    mfma_d4 acc[NT];
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt) acc[nt] = mfma_d4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int s = 0; s < KS; ++s) {
+      const double a = rts[cc][4 * s + kk];
+#pragma unroll
+      for (int nt = 0; nt < NT; ++nt)
+        acc[nt] = __builtin_amdgcn_mfma_f64_16x16x4f64(
+            a, (double)mv[s][nt], acc[nt], 0, 0, 0);
+    }
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int64_t g = gt + kk + 4 * r;
+        const int t = tw + 16 * nt + cc;
+        if (g < g1 && t < T) nu[(g * B + b) * (int64_t)T + t] = (float)acc[nt][r];
+      }
+    __syncthreads();
   }
 }
 
@@ -678,25 +755,27 @@ extern "C" int spr_grouped_spread_adjoint(int64_t P, int64_t D, int64_t B,
               "spr_grouped_spread_adjoint: null pointer");
   const int64_t nt = ceil_div(T, (int64_t)256);
   const int64_t nb = P * B * nt;
-  SPR_REQUIRE(nb < (int64_t)0x7fffffff,
+  SPR_REQUIRE(2 * nb < (int64_t)0x7fffffff,  // 128-bin blocks at D = 64
               "spr_grouped_spread_adjoint: grid too large");
   hipStream_t s = as_stream(stream);
-  if (DP <= 8)
+#define SPR_NU_MFMA(DPV)                                                      \
+  hipLaunchKernelGGL(                                                         \
+      (grouped_spread_adjoint_mfma_kernel<DPV>),                              \
+      dim3((unsigned)(P * B * ceil_div(T, (int64_t)SpreadAdjMfma<DPV>::BINS))), \
+      dim3(256), 0, s, (int)D, (int)B, (int)T,                                \
+      (int)ceil_div(T, (int64_t)SpreadAdjMfma<DPV>::BINS), grp_ptr, grp_key,  \
+      rtab, mu, nu)
+  if (DP == 16)
+    SPR_NU_MFMA(16);
+  else if (DP == 32)
+    SPR_NU_MFMA(32);
+  else if (DP == 64)
+    SPR_NU_MFMA(64);
+  else  // up to 8 directions: the vector kernel
     hipLaunchKernelGGL((grouped_spread_adjoint_kernel<8>), dim3((unsigned)nb),
                        dim3(256), 0, s, (int)D, (int)B, (int)T, (int)DP,
                        (int)nt, grp_ptr, grp_key, rtab, mu, nu);
-  else if (DP <= 16)
-    hipLaunchKernelGGL((grouped_spread_adjoint_kernel<16>), dim3((unsigned)nb),
-                       dim3(256), 0, s, (int)D, (int)B, (int)T, (int)DP,
-                       (int)nt, grp_ptr, grp_key, rtab, mu, nu);
-  else if (DP <= 32)
-    hipLaunchKernelGGL((grouped_spread_adjoint_kernel<32>), dim3((unsigned)nb),
-                       dim3(256), 0, s, (int)D, (int)B, (int)T, (int)DP,
-                       (int)nt, grp_ptr, grp_key, rtab, mu, nu);
-  else
-    hipLaunchKernelGGL((grouped_spread_adjoint_kernel<64>), dim3((unsigned)nb),
-                       dim3(256), 0, s, (int)D, (int)B, (int)T, (int)DP,
-                       (int)nt, grp_ptr, grp_key, rtab, mu, nu);
+#undef SPR_NU_MFMA
   SPR_LAUNCH_CHECK();
   return SPR_OK;
 }

@@ -86,6 +86,35 @@ def test_brdf_gradient_matches_fp64_reference(torch, every):
     assert err <= 1e-4
 
 
+@pytest.mark.parametrize("n_points", [(8, 8), (8, 16)])
+def test_brdf_gradient_many_directions(torch, n_points):
+    """32 and 64 directions (the matrix-core tiles of the table gradient and
+    of the transposed spread at their larger sizes): dL/dtable against the
+    FP64 reference."""
+    from sparrowpy_amd import adjoint, samplings
+    walls, _, _, _, _, _, src, rcv, (T, c, dt, K) = _setup()
+    dirs = samplings.hemisphere(samplings.sph_gaussian(n_points=n_points))
+    target = run(oracle_scene_general(walls, dirs,
+                                      directional_tables(dirs, len(walls), 3)),
+                 src, rcv, T, c, dt, K)
+    f0 = directional_tables(dirs, len(walls), 4)
+    ro = oracle_scene_general(walls, dirs, f0)
+    L, gH = loss_and_grad(run(ro, src, rcv, T, c, dt, K), target)
+    g_ref = oadj.gradient(ro, rcv, gH, K)
+    adj = _brdf_adjoint(gpu_scene_general(walls, dirs, f0), src, rcv, T, c,
+                        dt, K, 2)
+    assert adj.D == dirs.csize
+    table = adj.table0.clone().requires_grad_(True)
+    loss = adjoint.log_energy_loss(adjoint.receiver_histograms_brdf(adj,
+                                                                    table),
+                                   target)
+    loss.backward()
+    err = np.abs(table.grad.cpu().numpy() - g_ref).max() / np.abs(g_ref).max()
+    print(f"[brdf adjoint D={dirs.csize}] max grad err / max grad {err:.2e}")
+    assert abs(float(loss) - L) <= 1e-5 * L
+    assert err <= 1e-4
+
+
 def test_brdf_adjoint_agrees_with_diffuse_specular_adjoint(torch):
     """On create_from_scattering BRDFs both adjoints apply: the table
     gradient chained to (alpha, s) equals the diffuse + specular adjoint's
