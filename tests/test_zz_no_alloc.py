@@ -4,8 +4,10 @@ The entry point runs the kernels of spr_energy_exchange on a scratch array
 built into the library (one per GPU) instead of a workspace argument.  Its
 histograms must equal spr_energy_exchange's bit for bit (same kernels, same
 order of operations), on two calls in a row where the second scene is
-smaller (the scratch keeps the first call's contents), and stay within the
-exchange tolerance of the FP64 oracle.  Sizes beyond the built-in scratch
+smaller (the scratch keeps the first call's contents), run the same plan
+(the grouped exchange on the first scene's factorising F~ rows, the
+per-pair plan on the second's), and stay within the exchange tolerance of
+the FP64 oracle.  Sizes beyond the built-in scratch
 are rejected before any device work.
 """
 import ctypes
@@ -80,11 +82,13 @@ def _run(hip, name, r, T, c, dt, K):
 
 
 CASES = [
-    # the larger scene first, then a smaller one on the same scratch
+    # the larger scene first, then a smaller one on the same scratch; the
+    # first's F~ rows factorise into few groups (grouped exchange), the
+    # second's groups exceed a quarter of its pairs (per-pair plan)
     dict(dims=(3, 2.5, 2), patch=.5, source=[1.1, .8, .6], n_bands=2,
-         seed=0, T=400, dt=1 / 2000, K=8),
+         seed=0, T=400, dt=1 / 2000, K=8, grouped=True),
     dict(dims=(2, 2, 1), patch=.5, source=[.7, 1.2, .4], n_bands=1,
-         seed=1, T=150, dt=1 / 1000, K=5),
+         seed=1, T=150, dt=1 / 1000, K=5, grouped=False),
 ]
 
 
@@ -95,7 +99,12 @@ def test_noalloc_matches_workspace_entry_point(hip):
                    case["n_bands"], case["seed"])
         T, dt, K = case["T"], case["dt"], case["K"]
         ws = _run(hip, "spr_energy_exchange", r, T, c, dt, K)
+        g_ws = int(hip.lib().spr_energy_exchange_groups())
         na = _run(hip, "spr_energy_exchange_noalloc", r, T, c, dt, K)
+        g_na = int(hip.lib().spr_energy_exchange_groups())
+        # both ran the same plan: grouped (groups > 0) or per pair (0)
+        assert g_na == g_ws
+        assert (g_ws > 0) == case["grouped"]
         assert np.isfinite(na).all()
         np.testing.assert_array_equal(na, ws)
         ref = orc.exchange(T, r.energy_init_source,
